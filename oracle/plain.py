@@ -1,0 +1,312 @@
+"""Plain-PyTorch restatement of the reference render-and-optimise chain (oracle; test infrastructure).
+
+Every function follows the reference op for op -- same torch ops, same order, same dtypes --
+so that, run on the CPU in float32, it reproduces the reference bit for bit where the
+reference is deterministic.  It adds explicit inputs where the reference draws hidden
+randomness (per-ray camera index and uniforms instead of ``torch.rand``) and the opt-in
+extensions the benchmark scenes need (``near`` offset, occupancy skip, early termination),
+whose defaults reproduce the reference exactly.
+
+Division semantics.  ``normalize_samples_for_indecies`` divides by a Python float
+(src/ray_sampling.py:13).  ATen's CPU kernel divides; its GPU kernel multiplies by the
+float32 reciprocal of the divisor.  ``div_mode="divide"`` restates the former,
+``div_mode="reciprocal"`` the latter; both are pinned in tests (the second on the GPU).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and bench.py's
+baseline legs, never by the product package.
+"""
+import math
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+
+# --------------------------------------------------------------------------- grid layout
+
+def generate_grid(sx, sy, sz, points_distance=0.5, info_size=4, device="cpu"):
+    """src/grid_functions.py:184-217.
+
+    Returns (grid_coords (XYZ,3) f32, grid_cells (X,Y,Z,C) f32 zeros requiring grad,
+    meshgrid (X,Y,Z,3) int64, grid_grid (X,Y,Z,3) f32).  Cell (i,j,k) sits at world
+    ((i,j,k) - ceil(S/2) + 1) * pd.
+    Pinned by: closed form (tests/test_oracle.py::test_generate_grid_closed_form) and the
+    reference golden vector (tests/golden).
+    """
+    gx, gy, gz = (torch.arange(s, device=device) for s in (sx, sy, sz))
+    cx, cy, cz = torch.meshgrid(gx, gy, gz, indexing="ij")
+    meshgrid = torch.stack([cx, cy, cz], dim=-1)
+    cx = cx - float(np.ceil(sx / 2)) + 1
+    cy = cy - float(np.ceil(sy / 2)) + 1
+    cz = cz - float(np.ceil(sz / 2)) + 1
+    cx, cy, cz = cx * points_distance, cy * points_distance, cz * points_distance
+    grid_grid = torch.stack([cx, cy, cz], dim=-1)
+    grid_coords = grid_grid.reshape(sx * sy * sz, 3)
+    grid_cells = torch.zeros([sx, sy, sz, info_size], requires_grad=True, device=device)
+    return grid_coords, grid_cells, meshgrid, grid_grid
+
+
+def grid_min(grid_indices):
+    """The ``grid_indices.min(0)[0]`` term of src/ray_sampling.py:13 (min is order-exact)."""
+    return grid_indices.min(0)[0]
+
+
+def reciprocal_f32(points_distance):
+    """float32(1 / float32(pd)): the multiplier ATen's GPU true-division uses for a scalar
+    divisor (ATen BinaryDivTrueKernel.cu, outside the reference).  Pinned on the GPU by
+    tests/test_gpu_semantics.py."""
+    return float(np.float32(1.0) / np.float32(points_distance))
+
+
+# --------------------------------------------------------------------------- rays
+
+def camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v):
+    """Per-ray form of src/ray_sampling.py:212-264 (generate_rays_batched).
+
+    imgs (B,H,W,4), c2w (B,4,4), cam_ids (R,) int64, u/v (R,) in [0,1].  Returns
+    origins (R,3) = c2w[cam,:3,3] (src/ray_sampling.py:159), unit directions (R,3) and
+    the target pixels (R,4) imgs[cam, min(round(W*v),W-1), min(round(H*u),H-1)].
+    Elementwise ops are applied per ray in the reference's order, so the values equal
+    the reference's (B,R)-shaped computation for the same (cam, u, v).
+    Pinned by: reference golden vectors (generate_rays_batched), closed forms
+    (unit norm, centre ray = -Z axis, pixel round/clamp).
+    """
+    cam_x = c2w[:, :3, 0]
+    cam_y = c2w[:, :3, 1]
+    cam_z = -c2w[:, :3, 2]
+    aspect = cam_x.norm(dim=1) / cam_y.norm(dim=1)
+    H, W = imgs.shape[1], imgs.shape[2]
+    col = (H * u).round().clamp(max=H - 1).to(torch.long)
+    row = (W * v).round().clamp(max=W - 1).to(torch.long)
+    targets = imgs[cam_ids, row, col]
+    us = camera_angle_x * (u - 0.5)
+    vs = -((camera_angle_x * (1 / aspect))[cam_ids] * (v - 0.5))
+    d = us.unsqueeze(-1) * cam_x[cam_ids] + vs.unsqueeze(-1) * cam_y[cam_ids] + cam_z[cam_ids]
+    d = d / d.norm(dim=1).unsqueeze(-1)
+    origins = c2w[cam_ids, :3, 3]
+    return origins, d, targets
+
+
+def even_spread_uniforms(number_of_rays, num_cameras, device="cpu"):
+    """src/ray_sampling.py:220-223: cartesian_prod(linspace(0,1,sqrt R)) repeated per camera."""
+    n = int(np.round(np.sqrt(number_of_rays)))
+    ls = torch.linspace(0, end=1, steps=n, device=device)
+    return torch.cartesian_prod(ls, ls).repeat([num_cameras, 1, 1])
+
+
+def generate_rays_batched(imgs, number_of_rays, c2w, camera_angle_x, uniforms):
+    """src/ray_sampling.py:195-264 with the uniforms (B,R,2) passed in instead of drawn.
+
+    Returns (dirs (B*R,3), pixels_to_rays (B*R,4)) in camera-major order.
+    """
+    B, R = uniforms.shape[0], uniforms.shape[1]
+    cam_ids = torch.repeat_interleave(torch.arange(B, device=imgs.device), R, 0)
+    u = uniforms[:, :, 0].reshape(-1)
+    v = uniforms[:, :, 1].reshape(-1)
+    _, d, tgt = camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v)
+    return d, tgt
+
+
+# --------------------------------------------------------------------------- samples
+
+def sample_positions(origins, dirs, num_samples, delta_step, near=0.0):
+    """src/ray_sampling.py:161-167: p = o + d * t_k, t_k = delta*k (+ near), k = 1..S.
+
+    Layout (R*S,3), sample-fastest.  near=0 reproduces the reference exactly
+    (t + 0.0 == t).  Pinned by closed form ||p - o|| = t_k and golden vectors.
+    """
+    R = origins.shape[0]
+    k = torch.arange(num_samples + 1, device=origins.device, dtype=origins.dtype)[1:]
+    t = delta_step * k
+    if near:
+        t = t + near
+    t = t.repeat(R).unsqueeze(1)
+    o = torch.repeat_interleave(origins, num_samples, 0)
+    d = torch.repeat_interleave(dirs, num_samples, 0)
+    return o + d * t
+
+
+def normalize_samples(samples, gmin, points_distance, div_mode="divide"):
+    """src/ray_sampling.py:12-13 with grid_indices.min hoisted (gmin)."""
+    diff = samples - gmin
+    if div_mode == "divide":
+        return diff / points_distance
+    if div_mode == "reciprocal":
+        return diff * reciprocal_f32(points_distance)
+    raise ValueError(div_mode)
+
+
+# --------------------------------------------------------------------------- gathers
+
+def find_out_of_bound(indices, dims):
+    """src/grid_functions.py:47-63 (True = inside)."""
+    X, Y, Z = dims
+    i0, i1, i2 = indices[:, 0], indices[:, 1], indices[:, 2]
+    return (i0 < X) & (i0 >= 0) & (i1 < Y) & (i1 >= 0) & (i2 < Z) & (i2 >= 0)
+
+
+def wrap_indices(indices, dims):
+    """src/grid_functions.py:66-79: periodic wrap, sign of the divisor (-1 -> X-1)."""
+    out = indices.clone()
+    for a in range(3):
+        out[:, a] %= dims[a]
+    return out
+
+
+def nearest_values(n, grid, occupancy=None):
+    """src/grid_functions.py:103-114 composed as scripts/train.py:146-147.
+
+    idx = round-half-even(n) -> int64; in-bounds mask on the rounded index; wrap; gather
+    from grid.clip(0,1); multiply by the mask.  occupancy (X,Y,Z) bool, optional: a sample
+    whose nearest cell is unoccupied is skipped (value 0, no gradient).
+    Returns (values (N,C), keep mask (N,)).
+    """
+    dims = grid.shape[:3]
+    idx = torch.round(n).to(torch.long)
+    mask = find_out_of_bound(idx, dims)
+    w = wrap_indices(idx, dims)
+    vals = grid.clip(0, 1)[w[:, 0], w[:, 1], w[:, 2]]
+    if occupancy is not None:
+        mask = mask & occupancy[w[:, 0], w[:, 1], w[:, 2]]
+    return vals * mask.unsqueeze(-1), mask
+
+
+def get_grid_points_indices(n):
+    """src/grid_functions.py:220-246: (N,8,3) int64 corners, order x{ceil,floor} x y{..} x z{..}."""
+    cx, cy, cz = (torch.ceil(n[:, a]).unsqueeze(1) for a in range(3))
+    fx, fy, fz = (torch.floor(n[:, a]).unsqueeze(1) for a in range(3))
+    pts = [torch.cat([x, y, z], 1).unsqueeze(1)
+           for x in (cx, fx) for y in (cy, fy) for z in (cz, fz)]
+    return torch.cat(pts, 1).type(torch.long)
+
+
+def trilinear_interpolation(n, corners, grid):
+    """src/grid_functions.py:7-44 (signed frac weights; corners must be pre-wrapped)."""
+    f = torch.frac(n)
+    s0 = grid[corners[:, 4:, 0], corners[:, 4:, 1], corners[:, 4:, 2]]
+    s1 = grid[corners[:, :4, 0], corners[:, :4, 1], corners[:, :4, 2]]
+    fs = f[:, 0].unsqueeze(-1).unsqueeze(-1)
+    v = (s1 * fs) + (s0 * (1 - fs))
+    fs = f[:, 1].unsqueeze(-1).unsqueeze(-1)
+    v = (v[:, :2] * fs) + (v[:, 2:] * (1 - fs))
+    fs = f[:, 2].unsqueeze(-1)
+    return (v[:, 0] * fs) + (v[:, 1] * (1 - fs))
+
+
+def trilinear_values(n, grid, cell_occupancy=None):
+    """Trilinear composition of src/grid_functions.py:154-170 and :7-44 in the training chain.
+
+    Float in-bounds mask on the unrounded coordinate (0 <= n < X per axis, :162), corners
+    wrapped (:168), interpolation of grid.clip(0,1), times the mask.  cell_occupancy
+    (X,Y,Z) bool, optional: a sample whose floor cell (corner 7, wrapped) is unoccupied is
+    skipped.  Returns (values (N,C), keep mask (N,)).
+    """
+    dims = grid.shape[:3]
+    mask = find_out_of_bound(n, dims)
+    c = get_grid_points_indices(n)
+    N = c.shape[0]
+    c = wrap_indices(c.reshape(N * 8, 3), dims).reshape(N, 8, 3)
+    vals = trilinear_interpolation(n, c, grid.clip(0, 1))
+    if cell_occupancy is not None:
+        mask = mask & cell_occupancy[c[:, 7, 0], c[:, 7, 1], c[:, 7, 2]]
+    return vals * mask.unsqueeze(-1), mask
+
+
+def collect_cell_information_via_indices(n, grid):
+    """src/grid_functions.py:154-170: (N*8,C) gathered corners and the float in-bounds mask."""
+    dims = grid.shape[:3]
+    mask = find_out_of_bound(n, dims)
+    c = get_grid_points_indices(n)
+    c = wrap_indices(c.reshape(c.shape[0] * c.shape[1], 3), dims)
+    return grid[c[:, 0], c[:, 1], c[:, 2]], mask
+
+
+# --------------------------------------------------------------------------- compositing
+
+def compute_alpha_weighted_pixels(samples, t_stop=0.0):
+    """src/ray_sampling.py:172-192 on (..., S, 4): T = cumprod(1 - [0, a_0..a_{S-2}]),
+    w = a*T, RGB = sum w*c, A = sum w.  t_stop > 0 (extension) drops every sample whose
+    transmittance T_i is below t_stop (value and gradient zero).
+    """
+    if t_stop > 0:
+        with torch.no_grad():
+            a = samples[..., -1]
+            pad = torch.cat([torch.ones_like(a[..., :1]), 1 - a[..., :-1]], dim=-1)
+            keep = pad.cumprod(dim=-1) >= t_stop
+        samples = samples * keep.unsqueeze(-1)
+    alpha = samples[..., -1]
+    padded = torch.cat([torch.zeros_like(alpha[..., :1]), alpha], dim=-1)
+    cumprod = (1 - padded[..., :-1]).cumprod(dim=-1)
+    weights = (alpha * cumprod).unsqueeze(-1)
+    rgb = (samples[..., :-1] * weights).sum(-2)
+    final_alpha = weights.sum(dim=-2)
+    return torch.cat([rgb, final_alpha], dim=-1)
+
+
+# --------------------------------------------------------------------------- full chain
+
+def render(grid, origins, dirs, *, num_samples, delta_step, gmin, points_distance,
+           interp="nearest", near=0.0, div_mode="divide", occupancy=None, t_stop=0.0):
+    """The training chain scripts/train.py:130-153 on given rays: (R,4) RGBA.
+
+    grid is the raw (X,Y,Z,4) parameter (clipped inside).  For interp="trilinear" the
+    occupancy is the per-cell mask indexed by the floor cell.
+    """
+    R = origins.shape[0]
+    p = sample_positions(origins, dirs, num_samples, delta_step, near)
+    n = normalize_samples(p, gmin, points_distance, div_mode)
+    if interp == "nearest":
+        vals, _ = nearest_values(n, grid, occupancy)
+    elif interp == "trilinear":
+        vals, _ = trilinear_values(n, grid, occupancy)
+    else:
+        raise ValueError(interp)
+    vals = vals.reshape(R, num_samples, grid.shape[-1])
+    return compute_alpha_weighted_pixels(vals, t_stop)
+
+
+def render_chunked_backward(grid, origins, dirs, targets, *, chunk_rays, **kw):
+    """Forward + mean-MSE (scripts/train.py:156) + autograd backward over ray chunks.
+
+    The loss is normalised by the full batch (4 * R), exactly as one mse_loss over all rays,
+    so grid.grad equals the unchunked gradient up to summation order.  Returns (rgba, loss).
+    """
+    R = origins.shape[0]
+    outs = []
+    loss = 0.0
+    for s in range(0, R, chunk_rays):
+        e = min(R, s + chunk_rays)
+        out = render(grid, origins[s:e], dirs[s:e], **kw)
+        l = ((out - targets[s:e]) ** 2).sum() / (4 * R)
+        l.backward()
+        loss += float(l.detach())
+        outs.append(out.detach())
+    return torch.cat(outs, 0), loss
+
+
+def mse_loss(pred, target):
+    """scripts/train.py:156 (reduction = mean)."""
+    return F.mse_loss(pred, target)
+
+
+# --------------------------------------------------------------------------- occupancy
+
+def occupancy_to_bits(occ):
+    """(X,Y,Z) bool -> packed uint32 words, bit (L & 31) of word L >> 5, L = (i*Y+j)*Z+k."""
+    flat = occ.reshape(-1).to(torch.uint8).cpu().numpy()
+    pad = (-flat.size) % 32
+    flat = np.concatenate([flat, np.zeros(pad, np.uint8)])
+    bits = np.packbits(flat.reshape(-1, 32)[:, ::-1], axis=1, bitorder="big")
+    words = bits.view(">u4").astype(np.uint32).reshape(-1)
+    return torch.from_numpy(words.view(np.int32).copy())
+
+
+def dilate_cells(occ):
+    """Cell occupancy for trilinear skipping: cell (i,j,k) is occupied iff any of its 8
+    corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped, is occupied."""
+    out = occ.clone()
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                out |= torch.roll(occ, shifts=(-a, -b, -c), dims=(0, 1, 2))
+    return out
