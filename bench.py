@@ -1,0 +1,295 @@
+#!/usr/bin/env python
+"""Benchmark of the fused render-and-optimise step on one MI355X.
+
+Metric: fused fwd+bwd rays/s at a 256^3 grid, beside the plain PyTorch-ROCm path on the
+same GPU, with the dominant kernel's fraction of HBM peak.
+
+Workload "sparse256": 256^3 FP32 grid (blob field inside ~5% value-noise occupancy), 2^18
+rays drawn uniformly from 100 cameras (800x800 images resident in HBM, focal 1111 px) on
+the upper hemisphere of radius 4, 256 samples per ray from t = 2 + 4k/256, trilinear
+lookup, occupancy skipping, early termination at transmittance 1e-4.  Synthetic, seeded
+(grid 0, cameras 1, ray uniforms 2, targets 3).
+
+One step = zero-fill of the FP32 grid gradient + pxl_train_step (ray generation from
+(camera, u, v), target gather, sample placement, gather, compositing, MSE gradient,
+backward scatter) + the loss reduction.  Adam is not part of the step (reference
+scripts/train.py:182 runs it after the timed chain).
+
+Baselines (outside the timed region of the fused path):
+  plain_gpu     the plain PyTorch chain (oracle.plain, the reference's op sequence with
+                autograd backward) on the same GPU, same rays and semantics, in ray chunks
+  cpu_baseline  the same plain chain on the host CPU on the first --cpu-rays rays
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from plenoxels_amd import (CameraRays, MarchConfig, _lib, camera_rays, fill_,  # noqa: E402
+                           occupancy_dilate, pack_occupancy, train_step)
+from plenoxels_amd import scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fused fwd+bwd rays/s at 256^3 grid vs plain PyTorch-ROCm path; % HBM peak"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--interp", default="trilinear", choices=["trilinear", "nearest"])
+    ap.add_argument("--dtype", default="float32", choices=["float32", "bfloat16", "float16"])
+    ap.add_argument("--rays", type=int, default=1 << 18)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-chunk", type=int, default=32768)
+    ap.add_argument("--cpu-rays", type=int, default=2048)
+    ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-accounting", action="store_true")
+    return ap.parse_args()
+
+
+class Setup:
+    """Device-resident inputs of the sparse256 workload."""
+
+    def __init__(self, args, dev):
+        t0 = time.time()
+        w = scenes.make_workload("sparse256", n_rays=args.rays, interp=args.interp)
+        self.w = w
+        self.dev = dev
+        dtype = getattr(torch, args.dtype)
+        self.grid32 = w.grid.to(dev)
+        self.grid = self.grid32 if dtype == torch.float32 else self.grid32.to(dtype)
+        pts = pack_occupancy(w.occupancy.to(dev))
+        self.occ_bits = occupancy_dilate(pts, w.dims) if w.interp == "trilinear" else pts
+        B = w.c2w.shape[0]
+        H, W = w.img_hw
+        gen = torch.Generator(device=dev).manual_seed(3)
+        self.imgs = torch.rand(B, H, W, 4, device=dev, generator=gen)
+        self.camera = CameraRays(w.cam_ids.to(dev), w.uv.to(dev), w.c2w.to(dev),
+                                 w.camera_angle_x, self.imgs)
+        self.cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance,
+                               w.interp, w.near, w.t_stop, "reciprocal")
+        R = w.cam_ids.shape[0]
+        self.R = R
+        self.grad = torch.zeros(self.grid.shape, dtype=torch.float32, device=dev)
+        self.out = torch.empty(R, 4, device=dev)
+        self.sq = torch.empty(R, device=dev)
+        torch.cuda.synchronize()
+        log(f"setup {time.time() - t0:.1f}s: grid {tuple(self.grid.shape)} {self.grid.dtype}, "
+            f"occupancy {w.occupancy.float().mean().item():.4f}, rays {R}, S {w.num_samples}, {w.interp}")
+
+    def step(self, k_start=None, k_end=None):
+        fill_(self.grad, 0.0)
+        if k_start is not None:
+            k_start.record()
+        train_step(self.grid, self.grad, cfg=self.cfg, camera=self.camera, occupancy=self.occ_bits,
+                   out=self.out, ray_sq_err=self.sq)
+        if k_end is not None:
+            k_end.record()
+        return self.sq.sum() / (4 * self.R)
+
+
+def timed_fused(S, steps, warmup):
+    for _ in range(warmup):
+        S.step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        S.step(*ev[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    return wall, float(np.mean(kern_ms)), float(np.min(kern_ms))
+
+
+def algorithmic_bytes(S, args):
+    """Per-launch algorithmic bytes of pxl_train_step: per ray 48 B (cam id 4 + uv 8 in,
+    target 16, RGBA 16 out, squared error 4) plus K_r cells x 4 channels x (s_g read + 8 for
+    the FP32 gradient read-modify-write).  K_r counted exactly on the host from the seeded
+    batch (pxl_count_cells_host) using the termination counts of the forward kernel."""
+    w = S.w
+    L = _lib.lib()
+    o, d, _ = camera_rays(S.camera, want_targets=False)
+    aux = torch.empty(S.R, int(L.pxl_aux_floats(w.num_samples)), device=S.dev)
+    out = torch.empty(S.R, 4, device=S.dev)
+    from plenoxels_amd.render import _grid_struct, _rays_struct
+    g = _grid_struct(S.grid, S.occ_bits)
+    m = S.cfg.to_struct()
+    r = _rays_struct(o, d)
+    _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), _lib.ptr(out),
+                                _lib.ptr(aux), _lib.stream_handle(S.dev)))
+    nproc = aux[:, 0].contiguous().view(torch.int32).cpu().numpy().copy()
+    occ_host = S.occ_bits.cpu().numpy().view(np.uint32).copy()
+    gh = _lib.Grid()
+    for a in range(3):
+        gh.dims[a] = w.dims[a]
+    gh.channels = 4
+    gh.occupancy = occ_host.ctypes.data
+    oh = o.cpu().numpy().copy()
+    dh = d.cpu().numpy().copy()
+    counts = np.zeros(S.R, np.int64)
+    nt = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    _lib.check(L.pxl_count_cells_host(ctypes.byref(gh), ctypes.byref(m), oh.ctypes.data, dh.ctypes.data,
+                                      S.R, nproc.ctypes.data, counts.ctypes.data, nt))
+    s_g = S.grid.element_size()
+    per_cell = 4 * (s_g + 8)
+    total = S.R * 48 + int(counts.sum()) * per_cell
+    log(f"accounting {time.time() - t0:.1f}s: mean processed samples/ray {nproc.mean():.1f}, "
+        f"mean distinct cells/ray {counts.mean():.1f}")
+    return total, float(nproc.mean()), float(counts.mean())
+
+
+def plain_gpu(S, args):
+    """Plain PyTorch chain on the same GPU: rays -> samples -> gather -> composite -> MSE ->
+    autograd backward into grid.grad, chunked over rays, same semantics as the fused step."""
+    from oracle import plain
+    w = S.w
+    dev = S.dev
+    occ = w.occupancy.to(dev)
+    occ_used = plain.dilate_cells(occ) if w.interp == "trilinear" else occ
+    cam = w.cam_ids.to(dev).long()
+    u, v = w.uv[:, 0].to(dev).contiguous(), w.uv[:, 1].to(dev).contiguous()
+    gmin = torch.tensor(w.grid_min, device=dev)
+    grid = S.grid32.clone().requires_grad_(True)
+
+    def run():
+        grid.grad = None
+        o, d, tgt = plain.camera_rays(S.imgs, S.camera.c2w, w.camera_angle_x, cam, u, v)
+        plain.render_chunked_backward(grid, o, d, tgt, chunk_rays=args.plain_chunk,
+                                      num_samples=w.num_samples, delta_step=w.delta_step,
+                                      gmin=gmin, points_distance=w.points_distance,
+                                      interp=w.interp, near=w.near, div_mode="reciprocal",
+                                      occupancy=occ_used, t_stop=w.t_stop)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.plain_steps
+    # agreement of the plain path with the fused step on this batch (same inputs)
+    S.step()
+    torch.cuda.synchronize()
+    gerr = ((grid.grad - S.grad).abs().max() / grid.grad.abs().max().clamp_min(1e-30)).item()
+    return S.R / dt, dt, gerr
+
+
+def cpu_baseline(S, args):
+    """The plain chain on the host CPU for the first --cpu-rays rays (kind 'port': the
+    oracle's restatement of the reference ops)."""
+    from oracle import plain
+    w = S.w
+    n = min(args.cpu_rays, S.R)
+    cores = min(16, os.cpu_count() or 1)
+    torch.set_num_threads(cores)
+    occ = w.occupancy
+    occ_used = plain.dilate_cells(occ) if w.interp == "trilinear" else occ
+    grid = w.grid.clone().requires_grad_(True)
+    cam = w.cam_ids[:n].long()
+    u, v = w.uv[:n, 0].contiguous(), w.uv[:n, 1].contiguous()
+    imgs = S.imgs[: w.c2w.shape[0]].cpu()
+    gmin = torch.tensor(w.grid_min)
+
+    def run():
+        grid.grad = None
+        o, d, tgt = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam, u, v)
+        plain.render_chunked_backward(grid, o, d, tgt, chunk_rays=n, num_samples=w.num_samples,
+                                      delta_step=w.delta_step, gmin=gmin,
+                                      points_distance=w.points_distance, interp=w.interp,
+                                      near=w.near, div_mode="reciprocal", occupancy=occ_used,
+                                      t_stop=w.t_stop)
+    run()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": n / dt, "unit": "rays/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} rays of the {S.R}-ray sparse256 batch ({w.interp}, S={w.num_samples}, "
+                      f"fwd+bwd, plain PyTorch chain on CPU, mean of {reps} after 1 warm-up)"}
+
+
+def load_traffic(args):
+    """HBM bytes per launch of the train-step kernel from the committed PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_sparse256_{args.interp}_{args.dtype}.json")
+    if not os.path.exists(path) or args.rays != (1 << 18):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}); the path shards "
+              "as data parallelism over rays, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    S = Setup(args, dev)
+    wall, kern_ms, kern_min_ms = timed_fused(S, args.steps, args.warmup)
+    ms_per_step = wall * 1e3 / args.steps
+    value = S.R / (wall / args.steps)
+    log(f"fused: {ms_per_step:.3f} ms/step, train-step kernel {kern_ms:.3f} ms (min {kern_min_ms:.3f})")
+
+    roofline = None
+    extra = {}
+    if not args.no_accounting:
+        b_alg, mean_proc, mean_cells = algorithmic_bytes(S, args)
+        achieved = b_alg / (kern_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args),
+                    "kernel": "k_train_step", "kernel_ms": round(kern_ms, 4),
+                    "algorithmic_bytes": b_alg}
+        extra.update({"mean_processed_samples_per_ray": round(mean_proc, 2),
+                      "mean_distinct_cells_per_ray": round(mean_cells, 2)})
+    plain = None
+    cpu = None
+    if not args.no_baselines:
+        rps, dt, gerr = plain_gpu(S, args)
+        plain = {"value": round(rps, 1), "unit": "rays/s", "s_per_step": round(dt, 4),
+                 "grad_max_rel_diff_vs_fused": gerr}
+        log(f"plain GPU: {rps:.0f} rays/s ({dt * 1e3:.1f} ms/step), fused speed-up {value / rps:.1f}x")
+        cpu = cpu_baseline(S, args)
+        log(f"cpu baseline: {cpu['value']:.0f} rays/s on {cpu['cores']} threads")
+    rec = {
+        "metric": METRIC, "value": round(value, 1), "unit": "rays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"float32": "f32", "bfloat16": "bf16 storage, f32 compute",
+                  "float16": "f16 storage, f32 compute"}[args.dtype],
+        "data": "synthetic (seeded blob field, value-noise occupancy, random targets)",
+        "config": {"workload": "sparse256", "grid": [256, 256, 256], "rays": S.R, "samples_per_ray": 256,
+                   "interp": args.interp, "occupancy": "~5%", "t_stop": 1e-4, "cameras": 100,
+                   "image": [800, 800]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "speedup_vs_plain_gpu": round(value / plain["value"], 2) if plain else None,
+    }
+    rec.update(extra)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * pxl.h -- C ABI of libpxl, the MI355X (gfx950) kernels of the voxel-grid
+ * render-and-optimise path.
+ *
+ * The reference (DanJbk/Plenoxels) has no FFI: its boundary is the Python call signatures
+ * of src/ray_sampling.py and src/grid_functions.py as called from scripts/train.py:130-157
+ * (training, with autograd backward) and src/visualization.py:125-146 (forward-only render).
+ * Each entry point below names the reference function(s) whose work it replaces.
+ *
+ * Conventions
+ *   - Plain C: pointers are device pointers (hipMalloc'd or torch-owned), sizes are int64.
+ *   - The library never allocates device memory; the caller owns every buffer.
+ *   - Work is enqueued asynchronously on `stream` (a hipStream_t; NULL = default stream).
+ *   - Return 0 on success or a negative PXL_E_* code; pxl_last_error() then describes it
+ *     (thread-local).  Argument validation happens before any HIP call.
+ *   - Grid layout (src/grid_functions.py:214-215): row-major (X,Y,Z,4), channel innermost,
+ *     channels [R,G,B,alpha]; linear cell index L = (i*Y + j)*Z + k.  Gradients are always
+ *     FP32 (X,Y,Z,4) and are ACCUMULATED into (the caller zero-fills when it wants d/dgrid).
+ *   - Gradient accumulation uses float atomics: summation order, hence the last bits of
+ *     a gradient, may differ run to run.
+ */
+#ifndef PXL_H
+#define PXL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXL_ABI_VERSION 1
+
+/* return codes */
+#define PXL_OK 0
+#define PXL_E_INVALID (-1)   /* bad argument (null pointer, bad enum, negative size) */
+#define PXL_E_DTYPE (-2)     /* unsupported storage dtype / channel count */
+#define PXL_E_SIZE (-3)      /* a size exceeds what the kernels index (e.g. >= 2^31 cells) */
+#define PXL_E_HIP (-4)       /* a HIP runtime call or kernel launch failed */
+#define PXL_E_NO_DEVICE (-5) /* no HIP device available */
+
+/* grid storage dtypes */
+#define PXL_DTYPE_F32 0
+#define PXL_DTYPE_BF16 1
+#define PXL_DTYPE_F16 2
+
+/* sample-to-grid lookup */
+#define PXL_INTERP_NEAREST 0   /* src/grid_functions.py:103-114 */
+#define PXL_INTERP_TRILINEAR 1 /* src/grid_functions.py:7-44, :154-170, :220-246 */
+
+/* pxl_march.flags */
+#define PXL_FLAG_EXACT_DIV 1 /* n = (p - min)/pd (ATen CPU); default n = (p - min)*inv_pd (ATen GPU) */
+
+/* A dense voxel grid. */
+typedef struct pxl_grid {
+  const void* data;          /* (X,Y,Z,4) cells in `dtype` */
+  int64_t dims[3];           /* X, Y, Z */
+  int32_t channels;          /* must be 4 */
+  int32_t dtype;             /* PXL_DTYPE_* */
+  const uint32_t* occupancy; /* optional: bit (L & 31) of word L >> 5 set = cell L occupied.
+                                Nearest skips a sample whose nearest cell is unoccupied;
+                                trilinear skips one whose floor cell is unoccupied (see
+                                pxl_occupancy_dilate).  NULL = every cell occupied. */
+} pxl_grid;
+
+/* How samples are placed along each ray and mapped to grid-index space
+ * (src/ray_sampling.py:161-167 and :12-13):
+ *   t_k = delta*k + near, k = 1..n_samples;   p = o + d*t_k;   n = (p - grid_min) * inv_pd
+ * every step rounded to float32 separately, as the reference's torch ops are. */
+typedef struct pxl_march {
+  int32_t n_samples;  /* S >= 0 (<= 2^24) */
+  int32_t interp;     /* PXL_INTERP_* */
+  int32_t flags;      /* PXL_FLAG_* */
+  int32_t reserved;   /* 0 */
+  float delta;        /* delta_step */
+  float near;         /* 0 reproduces the reference */
+  float t_stop;       /* early termination: drop samples whose transmittance < t_stop; 0 = off */
+  float pd;           /* points_distance as float32 */
+  float inv_pd;       /* float32(1/float32(pd)) */
+  float grid_min[3];  /* grid_indices.min(0) (src/ray_sampling.py:13) */
+} pxl_march;
+
+/* A batch of rays, in one of two forms.
+ * Explicit: origins/dirs (n_rays,3) f32.
+ * Camera (origins == NULL): ray r comes from camera cam_ids[r] at uniforms uv[r] = (u,v) in
+ * [0,1], exactly as src/ray_sampling.py:212-264 derives a direction and a target pixel from
+ * a c2w matrix and camera_angle_x; its origin is c2w[cam][:3,3] (src/ray_sampling.py:159). */
+typedef struct pxl_rays {
+  int64_t n_rays;
+  const float* origins;   /* (n,3) or NULL for camera form */
+  const float* dirs;      /* (n,3) */
+  const int32_t* cam_ids; /* (n,)  camera form */
+  const float* uv;        /* (n,2) camera form */
+  const float* c2w;       /* (n_cams,4,4) row-major camera-to-world matrices */
+  int64_t n_cams;
+  float camera_angle_x;
+  int32_t reserved;
+  const float* imgs;      /* (n_cams,H,W,4) target images or NULL */
+  int64_t img_h, img_w;
+} pxl_rays;
+
+int pxl_abi_version(void);
+const char* pxl_last_error(void);
+
+/* Floats of per-ray state pxl_render_fwd saves for pxl_render_bwd. */
+int64_t pxl_aux_floats(int32_t n_samples);
+
+/* Fused forward render: sample placement, gather (clip to [0,1], out-of-grid samples
+ * zeroed), front-to-back compositing.  out_rgba (n,4) f32; aux (n, pxl_aux_floats(S)) f32
+ * or NULL when no backward will follow.
+ * Replaces, for one batch: src/ray_sampling.py:161-167 (sample_camera_rays_batched),
+ * :12-13 (normalize_samples_for_indecies), src/grid_functions.py:103-114
+ * (get_nearest_voxels) or :154-170 + :7-44 (trilinear), scripts/train.py:146-147 (clip,
+ * mask), src/ray_sampling.py:172-192 (compute_alpha_weighted_pixels). */
+int pxl_render_fwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   float* out_rgba, float* aux, void* stream);
+
+/* Fused backward of pxl_render_fwd: accumulates dL/dgrid into grad_grid (X,Y,Z,4) f32
+ * given dL/dout (n,4).  Replaces autograd's replay of the chain above
+ * (scripts/train.py:181, the backward of :142-153). */
+int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   const float* grad_rgba, const float* aux, float* grad_grid, void* stream);
+
+/* One training step's render + loss gradient + backward in one pass:
+ * out = render(rays); dL/dout = grad_scale * (out - target); accumulates dL/dgrid.
+ * targets (n,4) f32, or NULL in camera form with imgs set (targets gathered from imgs as
+ * src/ray_sampling.py:238-248 does).  grad_scale = 2/(4*N_total) reproduces mse_loss's
+ * mean (scripts/train.py:156).  ray_sq_err (n,) f32 or NULL: sum over channels of
+ * (out-target)^2 per ray.  out_rgba may be NULL.
+ * Replaces scripts/train.py:130-157 + :180-181 minus the optimiser. */
+int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   const float* targets, float grad_scale, float* out_rgba,
+                   float* ray_sq_err, float* grad_grid, void* stream);
+
+/* Ray generation only (src/ray_sampling.py:195-264 on per-ray camera/uniforms):
+ * origins (n,3), unit dirs (n,3), targets (n,4) (NULL to skip; needs imgs). */
+int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,
+                    void* stream);
+
+/* Cell occupancy for trilinear skipping: out bit L = OR of in bits of the 8 corners
+ * (i+a, j+b, k+c), a,b,c in {0,1}, wrapped.  out must not alias in; words = ceil(XYZ/32). */
+int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z,
+                         uint32_t* occ_out, void* stream);
+
+/* Pack a (X,Y,Z) occupancy grid of bytes (nonzero = occupied) into the bit layout of
+ * pxl_grid.occupancy: bit (L & 31) of word L >> 5.  out: ceil(n_cells/32) words. */
+int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream);
+
+/* dst[0..n) = value (HBM-rate fill, e.g. the per-step gradient zero-fill). */
+int pxl_fill_f32(float* dst, int64_t n, float value, void* stream);
+
+/* Host-side traffic accounting (no counterpart in the reference; ALL POINTERS ARE HOST
+ * POINTERS, including grid->occupancy, and grid->data is not read): counts[r] = number of
+ * distinct grid cells the processed samples of ray r read -- nearest cell, or the union of
+ * the 8 trilinear corners, of every in-grid, non-skipped sample k <= n_proc[r] (n_proc may
+ * be NULL = all samples; pxl_render_fwd stores it in aux[r][0]).  Multithreaded. */
+int pxl_count_cells_host(const pxl_grid* grid, const pxl_march* march, const float* origins,
+                         const float* dirs, int64_t n_rays, const int32_t* n_proc,
+                         int64_t* counts, int32_t n_threads);
+
+/* Test hook (no counterpart in the reference): runs the wave primitives the kernels are
+ * built from on one wavefront.  in: 6*64 floats (x, then affine-map fields a0,a1,a2,a3,b);
+ * out: 7*64 floats (inclusive product, exclusive product, wave sum, suffix-composed a0,
+ * a3, b, and x shifted down one lane with -1 filling lane 63). */
+int pxl_debug_wave_ops(const float* in, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PXL_H */

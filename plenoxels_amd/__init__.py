@@ -1,0 +1,12 @@
+"""plenoxels_amd -- MI355X-native voxel-grid render-and-optimise path.
+
+Drop-in for the hot path of DanJbk/Plenoxels (src/ray_sampling.py, src/grid_functions.py as
+called from scripts/train.py and src/visualization.py), computed by hand-written gfx950 HIP
+kernels in libpxl.so (C ABI: include/pxl.h).  There is no CPU fallback: every entry point
+raises if the library is not built or no HIP device is present.
+"""
+from .render import (CameraRays, MarchConfig, camera_rays, fill_, grid_min_of,  # noqa: F401
+                     occupancy_dilate, pack_occupancy, render_rays, train_step)
+
+__all__ = ["CameraRays", "MarchConfig", "camera_rays", "fill_", "grid_min_of",
+           "occupancy_dilate", "pack_occupancy", "render_rays", "train_step"]

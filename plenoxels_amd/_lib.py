@@ -1,0 +1,119 @@
+"""ctypes binding of libpxl (include/pxl.h).
+
+torch is imported first on purpose: torch ships its own HIP runtime (libamdhip64.so.7)
+and libpxl binds to that same already-loaded runtime through the shared soname, so the
+streams and device pointers torch hands over are valid in the library.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime libpxl binds to)
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libpxl.so")
+
+PXL_OK = 0
+PXL_E_INVALID = -1
+PXL_E_DTYPE = -2
+PXL_E_SIZE = -3
+PXL_E_HIP = -4
+PXL_E_NO_DEVICE = -5
+
+DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
+INTERP_NEAREST, INTERP_TRILINEAR = 0, 1
+FLAG_EXACT_DIV = 1
+
+c_void_p = ctypes.c_void_p
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("data", c_void_p), ("dims", ctypes.c_int64 * 3), ("channels", ctypes.c_int32),
+                ("dtype", ctypes.c_int32), ("occupancy", c_void_p)]
+
+
+class March(ctypes.Structure):
+    _fields_ = [("n_samples", ctypes.c_int32), ("interp", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("delta", ctypes.c_float), ("near", ctypes.c_float), ("t_stop", ctypes.c_float),
+                ("pd", ctypes.c_float), ("inv_pd", ctypes.c_float),
+                ("grid_min", ctypes.c_float * 3)]
+
+
+class Rays(ctypes.Structure):
+    _fields_ = [("n_rays", ctypes.c_int64), ("origins", c_void_p), ("dirs", c_void_p),
+                ("cam_ids", c_void_p), ("uv", c_void_p), ("c2w", c_void_p),
+                ("n_cams", ctypes.c_int64), ("camera_angle_x", ctypes.c_float),
+                ("reserved", ctypes.c_int32), ("imgs", c_void_p),
+                ("img_h", ctypes.c_int64), ("img_w", ctypes.c_int64)]
+
+
+_GP = ctypes.POINTER(Grid)
+_MP = ctypes.POINTER(March)
+_RP = ctypes.POINTER(Rays)
+
+# symbol -> (restype, argtypes); mirrors include/pxl.h
+SIGNATURES = {
+    "pxl_abi_version": (ctypes.c_int, []),
+    "pxl_last_error": (ctypes.c_char_p, []),
+    "pxl_aux_floats": (ctypes.c_int64, [ctypes.c_int32]),
+    "pxl_render_fwd": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, c_void_p, c_void_p]),
+    "pxl_render_bwd": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pxl_train_step": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, ctypes.c_float, c_void_p,
+                                      c_void_p, c_void_p, c_void_p]),
+    "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_fill_f32": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p]),
+    "pxl_count_cells_host": (ctypes.c_int, [_GP, _MP, c_void_p, c_void_p, ctypes.c_int64,
+                                            c_void_p, c_void_p, ctypes.c_int32]),
+    "pxl_debug_wave_ops": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+}
+
+
+class PxlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpxl error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libpxl.so (built in-tree by __graft_entry__.build()); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` from the repository root")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        if L.pxl_abi_version() != 1:
+            raise ImportError("libpxl ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().pxl_last_error().decode(errors="replace")
+
+
+def check(rc):
+    if rc != PXL_OK:
+        raise PxlError(rc, last_error())
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    """hipStream_t of torch's current stream on `device`."""
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

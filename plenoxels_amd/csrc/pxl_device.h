@@ -1,0 +1,442 @@
+// pxl_device.h -- device building blocks of the gfx950 render kernels.
+//
+// Everything here runs with one 64-lane wavefront per ray: lane l of chunk c handles
+// sample k = 64c + l + 1.  Cross-lane work uses DPP (row shifts, row broadcasts, wave
+// shifts) and v_readlane; all wave primitives must be called with every lane active.
+//
+// Floating-point order.  The translation unit is compiled with -ffp-contract=off, so each
+// `a*b + c` below rounds twice, exactly as the reference's separate torch ops do; an FMA
+// appears only where written as fmaf().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pxl.h"
+
+namespace pxl {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+// ----------------------------------------------------------------------------- DPP
+// GFX9/CDNA DPP control words.
+enum : int {
+  DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL2 = 0x102, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHL8 = 0x108,
+  DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118,
+  DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138,
+  DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143,
+};
+
+// Lane l receives `src` from the lane CTRL selects; lanes with no valid source (or rows
+// outside ROW_MASK) receive `old`.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src),
+                                                    CTRL, ROW_MASK, 0xf, false));
+}
+
+__device__ __forceinline__ float readlane(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// Inclusive product over lanes 0..l (Hillis-Steele within 16-lane rows, then row
+// broadcasts of lanes 15 and 31).
+__device__ __forceinline__ float wave_prefix_mul(float x) {
+  x *= dpp<DPP_ROW_SHR1>(1.f, x);
+  x *= dpp<DPP_ROW_SHR2>(1.f, x);
+  x *= dpp<DPP_ROW_SHR4>(1.f, x);
+  x *= dpp<DPP_ROW_SHR8>(1.f, x);
+  x *= dpp<DPP_ROW_BCAST15, 0xa>(1.f, x);
+  x *= dpp<DPP_ROW_BCAST31, 0xc>(1.f, x);
+  return x;
+}
+
+// Exclusive form of an inclusive scan: lane l gets lane l-1's value, lane 0 gets `ident`.
+__device__ __forceinline__ float wave_shift_up(float incl, float ident) {
+  return dpp<DPP_WAVE_SHR1>(ident, incl);
+}
+
+// Lane l gets lane l+1's value, lane 63 gets `fill`.
+__device__ __forceinline__ float wave_shift_down(float x, float fill) {
+  return dpp<DPP_WAVE_SHL1>(fill, x);
+}
+
+// Sum over all 64 lanes, result in every lane.
+__device__ __forceinline__ float wave_sum(float x) {
+  x += dpp<DPP_ROW_SHR1>(0.f, x);
+  x += dpp<DPP_ROW_SHR2>(0.f, x);
+  x += dpp<DPP_ROW_SHR4>(0.f, x);
+  x += dpp<DPP_ROW_SHR8>(0.f, x);
+  x += dpp<DPP_ROW_BCAST15, 0xa>(0.f, x);
+  x += dpp<DPP_ROW_BCAST31, 0xc>(0.f, x);
+  return readlane(x, 63);
+}
+
+// Suffix scan of affine maps f_l(q) = A_l + B_l q under composition f_l o f_{l+1} o ... o
+// f_63: on return lane l holds the composed map.  A is 4-wide (RGB and alpha), B scalar.
+// Within rows: row_shl Hillis-Steele; across rows: the row totals (lanes 16, 32, 48) are
+// read into scalars, composed, and applied as each row's tail.
+struct Affine4 {
+  float a0, a1, a2, a3, b;
+};
+
+template <int CTRL>
+__device__ __forceinline__ void affine_step(Affine4& f) {
+  const float a0 = dpp<CTRL>(0.f, f.a0), a1 = dpp<CTRL>(0.f, f.a1);
+  const float a2 = dpp<CTRL>(0.f, f.a2), a3 = dpp<CTRL>(0.f, f.a3);
+  const float b = dpp<CTRL>(1.f, f.b);
+  f.a0 = f.a0 + f.b * a0;
+  f.a1 = f.a1 + f.b * a1;
+  f.a2 = f.a2 + f.b * a2;
+  f.a3 = f.a3 + f.b * a3;
+  f.b = f.b * b;
+}
+
+__device__ __forceinline__ Affine4 affine_readlane(const Affine4& f, int lane) {
+  return Affine4{readlane(f.a0, lane), readlane(f.a1, lane), readlane(f.a2, lane),
+                 readlane(f.a3, lane), readlane(f.b, lane)};
+}
+
+// g o h
+__device__ __forceinline__ Affine4 affine_compose(const Affine4& g, const Affine4& h) {
+  return Affine4{g.a0 + g.b * h.a0, g.a1 + g.b * h.a1, g.a2 + g.b * h.a2, g.a3 + g.b * h.a3,
+                 g.b * h.b};
+}
+
+__device__ __forceinline__ void wave_suffix_affine(Affine4& f, int lane) {
+  affine_step<DPP_ROW_SHL1>(f);
+  affine_step<DPP_ROW_SHL2>(f);
+  affine_step<DPP_ROW_SHL4>(f);
+  affine_step<DPP_ROW_SHL8>(f);
+  const Affine4 r1 = affine_readlane(f, 16), r2 = affine_readlane(f, 32),
+                r3 = affine_readlane(f, 48);
+  const Affine4 t2 = r3;
+  const Affine4 t1 = affine_compose(r2, r3);
+  const Affine4 t0 = affine_compose(r1, t1);
+  const int row = lane >> 4;
+  Affine4 t = row == 0 ? t0 : (row == 1 ? t1 : t2);
+  if (row != 3) f = affine_compose(f, t);
+}
+
+// ----------------------------------------------------------------------------- grid cells
+
+template <int DT>
+__device__ __forceinline__ float4 load_cell(const void* grid, uint32_t cell);
+
+template <>
+__device__ __forceinline__ float4 load_cell<PXL_DTYPE_F32>(const void* grid, uint32_t cell) {
+  return reinterpret_cast<const float4*>(grid)[cell];
+}
+
+template <>
+__device__ __forceinline__ float4 load_cell<PXL_DTYPE_BF16>(const void* grid, uint32_t cell) {
+  const uint2 w = reinterpret_cast<const uint2*>(grid)[cell];
+  return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u),
+                     __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xffff0000u));
+}
+
+__device__ __forceinline__ float half_bits_to_float(uint32_t h) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+}
+
+template <>
+__device__ __forceinline__ float4 load_cell<PXL_DTYPE_F16>(const void* grid, uint32_t cell) {
+  const uint2 w = reinterpret_cast<const uint2*>(grid)[cell];
+  return make_float4(half_bits_to_float(w.x & 0xffffu), half_bits_to_float(w.x >> 16),
+                     half_bits_to_float(w.y & 0xffffu), half_bits_to_float(w.y >> 16));
+}
+
+__device__ __forceinline__ float clip01(float v) { return fminf(fmaxf(v, 0.f), 1.f); }
+
+// Bit c set when channel c lies in [0,1]: torch.clamp's gradient passes at both bounds.
+__device__ __forceinline__ uint32_t clip_bits(const float4& v) {
+  return (uint32_t)(v.x >= 0.f && v.x <= 1.f) | ((uint32_t)(v.y >= 0.f && v.y <= 1.f) << 1) |
+         ((uint32_t)(v.z >= 0.f && v.z <= 1.f) << 2) | ((uint32_t)(v.w >= 0.f && v.w <= 1.f) << 3);
+}
+
+__device__ __forceinline__ float4 clip4(const float4& v) {
+  return make_float4(clip01(v.x), clip01(v.y), clip01(v.z), clip01(v.w));
+}
+
+__device__ __forceinline__ bool occupied(const uint32_t* occ, uint32_t cell) {
+  return occ == nullptr || ((occ[cell >> 5] >> (cell & 31u)) & 1u);
+}
+
+// ----------------------------------------------------------------------------- parameters
+
+struct KParams {
+  const void* grid;
+  const uint32_t* occ;
+  int X, Y, Z, n_samples;
+  float fX, fY, fZ;
+  float delta, near, t_stop, pd, inv_pd;
+  float gx, gy, gz;
+  int exact_div;
+  int aux_stride;
+  int64_t n_rays;
+  int64_t ray_offset;
+  // rays
+  const float* origins;
+  const float* dirs;
+  const int32_t* cam_ids;
+  const float* uv;
+  const float* c2w;
+  const float* imgs;
+  int64_t n_cams, img_h, img_w;
+  float theta;
+};
+
+struct Ray {
+  float ox, oy, oz, dx, dy, dz;
+  int cam;
+  float u, v;
+};
+
+// three-element norm in the order torch's CPU reduction uses: sqrt(fma(z,z,fma(y,y,x*x))).
+// __builtin_sqrtf lowers to the correctly rounded sequence (v_sqrt_f32 plus the +-1 ulp
+// fma checks); __fsqrt_rn lowers to the bare v_sqrt_f32, which is not correctly rounded.
+__device__ __forceinline__ float norm3(float x, float y, float z) {
+  return __builtin_sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
+}
+
+// src/ray_sampling.py:212-264 for one ray of the camera form (every op rounded as torch does).
+__device__ __forceinline__ void camera_ray(const KParams& P, int64_t r, Ray& R) {
+  // an out-of-range camera index is clamped (the reference raises IndexError instead)
+  const int cam = (int)min(max((int64_t)P.cam_ids[r], (int64_t)0), P.n_cams - 1);
+  const float u = P.uv[2 * r], v = P.uv[2 * r + 1];
+  const float* M = P.c2w + (int64_t)cam * 16;
+  const float xx = M[0], xy = M[4], xz = M[8];
+  const float yx = M[1], yy = M[5], yz = M[9];
+  const float zx = -M[2], zy = -M[6], zz = -M[10];
+  const float aspect = __fdiv_rn(norm3(xx, xy, xz), norm3(yx, yy, yz));
+  const float us = P.theta * (u - 0.5f);
+  const float vs = -((P.theta * __fdiv_rn(1.f, aspect)) * (v - 0.5f));
+  const float dx = ((us * xx) + (vs * yx)) + zx;
+  const float dy = ((us * xy) + (vs * yy)) + zy;
+  const float dz = ((us * xz) + (vs * yz)) + zz;
+  const float nd = norm3(dx, dy, dz);
+  R.dx = __fdiv_rn(dx, nd);
+  R.dy = __fdiv_rn(dy, nd);
+  R.dz = __fdiv_rn(dz, nd);
+  R.ox = M[3];
+  R.oy = M[7];
+  R.oz = M[11];
+  R.cam = cam;
+  R.u = u;
+  R.v = v;
+}
+
+// Target pixel imgs[cam, min(round(W*v),W-1), min(round(H*u),H-1)] (src/ray_sampling.py:238-248;
+// rows from v, columns from u).  Indices are additionally clamped into the image, where the
+// reference would raise IndexError instead (only reachable for non-square images).
+__device__ __forceinline__ float4 camera_target(const KParams& P, const Ray& R) {
+  const float H = (float)P.img_h, W = (float)P.img_w;
+  float col = fminf(__builtin_rintf(H * R.u), H - 1.f);
+  float row = fminf(__builtin_rintf(W * R.v), W - 1.f);
+  col = fminf(fmaxf(col, 0.f), W - 1.f);
+  row = fminf(fmaxf(row, 0.f), H - 1.f);
+  const int64_t pix = ((int64_t)R.cam * P.img_h + (int64_t)row) * P.img_w + (int64_t)col;
+  return reinterpret_cast<const float4*>(P.imgs)[pix];
+}
+
+__device__ __forceinline__ void setup_ray(const KParams& P, int64_t r, Ray& R) {
+  if (P.origins != nullptr) {
+    R.ox = P.origins[3 * r];
+    R.oy = P.origins[3 * r + 1];
+    R.oz = P.origins[3 * r + 2];
+    R.dx = P.dirs[3 * r];
+    R.dy = P.dirs[3 * r + 1];
+    R.dz = P.dirs[3 * r + 2];
+    R.cam = 0;
+    R.u = R.v = 0.f;
+  } else {
+    camera_ray(P, r, R);
+  }
+}
+
+// ----------------------------------------------------------------------------- geometry
+
+__device__ __forceinline__ float to_index(const KParams& P, float p, float g) {
+  const float diff = p - g;
+  return P.exact_div ? __fdiv_rn(diff, P.pd) : diff * P.inv_pd;
+}
+
+// Continuous grid-index coordinate of sample k (src/ray_sampling.py:161-167, :13).
+__device__ __forceinline__ void sample_index(const KParams& P, const Ray& R, int k, float& nx,
+                                             float& ny, float& nz) {
+  float t = P.delta * (float)k;
+  t = t + P.near;
+  nx = to_index(P, R.ox + R.dx * t, P.gx);
+  ny = to_index(P, R.oy + R.dy * t, P.gy);
+  nz = to_index(P, R.oz + R.dz * t, P.gz);
+}
+
+// Nearest voxel: round half to even, integer in-bounds test (src/grid_functions.py:111-112).
+__device__ __forceinline__ bool nearest_cell(const KParams& P, float nx, float ny, float nz,
+                                             uint32_t& cell) {
+  const float rx = __builtin_rintf(nx), ry = __builtin_rintf(ny), rz = __builtin_rintf(nz);
+  const bool inb = rx >= 0.f && rx < P.fX && ry >= 0.f && ry < P.fY && rz >= 0.f && rz < P.fZ;
+  cell = inb ? ((uint32_t)rx * (uint32_t)P.Y + (uint32_t)ry) * (uint32_t)P.Z + (uint32_t)rz : 0u;
+  return inb;
+}
+
+// Trilinear cell: float in-bounds test on the unrounded coordinate (src/grid_functions.py:162),
+// ceil/floor corners wrapped (:168; ceil == X wraps to 0), fractions n - trunc(n) (:29).
+struct TriCell {
+  uint32_t xs[2], ys[2], zs[2];  // [0] = ceil, [1] = floor; xs pre-multiplied by Y*Z, ys by Z
+  float fx, fy, fz;
+};
+
+__device__ __forceinline__ bool trilinear_cell(const KParams& P, float nx, float ny, float nz,
+                                               TriCell& t) {
+  const bool inb = nx >= 0.f && nx < P.fX && ny >= 0.f && ny < P.fY && nz >= 0.f && nz < P.fZ;
+  if (!inb) return false;
+  const uint32_t x0 = (uint32_t)floorf(nx), y0 = (uint32_t)floorf(ny), z0 = (uint32_t)floorf(nz);
+  uint32_t x1 = (uint32_t)ceilf(nx), y1 = (uint32_t)ceilf(ny), z1 = (uint32_t)ceilf(nz);
+  x1 = x1 >= (uint32_t)P.X ? x1 - (uint32_t)P.X : x1;
+  y1 = y1 >= (uint32_t)P.Y ? y1 - (uint32_t)P.Y : y1;
+  z1 = z1 >= (uint32_t)P.Z ? z1 - (uint32_t)P.Z : z1;
+  const uint32_t yz = (uint32_t)P.Y * (uint32_t)P.Z;
+  t.xs[0] = x1 * yz;
+  t.xs[1] = x0 * yz;
+  t.ys[0] = y1 * (uint32_t)P.Z;
+  t.ys[1] = y0 * (uint32_t)P.Z;
+  t.zs[0] = z1;
+  t.zs[1] = z0;
+  t.fx = nx - truncf(nx);
+  t.fy = ny - truncf(ny);
+  t.fz = nz - truncf(nz);
+  return true;
+}
+
+// ----------------------------------------------------------------------------- samples
+
+struct Sample {
+  float4 v;       // clipped (and interpolated) value; zero unless keep
+  uint32_t clip;  // clip-gradient bits: nearest bit c; trilinear bit 4j+c for corner j
+  bool keep;
+};
+
+template <int DT>
+__device__ __forceinline__ void eval_nearest(const KParams& P, const Ray& R, int k, bool active,
+                                             Sample& s) {
+  s.v = make_float4(0.f, 0.f, 0.f, 0.f);
+  s.clip = 0u;
+  s.keep = false;
+  if (!active) return;
+  float nx, ny, nz;
+  sample_index(P, R, k, nx, ny, nz);
+  uint32_t cell;
+  if (!nearest_cell(P, nx, ny, nz, cell) || !occupied(P.occ, cell)) return;
+  const float4 raw = load_cell<DT>(P.grid, cell);
+  s.v = clip4(raw);
+  s.clip = clip_bits(raw);
+  s.keep = true;
+}
+
+// Corner j = 4*bx + 2*by + bz, b = 0 ceil / 1 floor: the order of
+// src/grid_functions.py:238-243, which trilinear_interpolation (:31-42) consumes.
+template <int DT>
+__device__ __forceinline__ void eval_trilinear(const KParams& P, const Ray& R, int k,
+                                               bool active, Sample& s) {
+  s.v = make_float4(0.f, 0.f, 0.f, 0.f);
+  s.clip = 0u;
+  s.keep = false;
+  if (!active) return;
+  float nx, ny, nz;
+  sample_index(P, R, k, nx, ny, nz);
+  TriCell t;
+  if (!trilinear_cell(P, nx, ny, nz, t)) return;
+  if (!occupied(P.occ, t.xs[1] + t.ys[1] + t.zs[1])) return;
+  float4 c[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const float4 raw = load_cell<DT>(P.grid, t.xs[j >> 2] + t.ys[(j >> 1) & 1] + t.zs[j & 1]);
+    s.clip |= clip_bits(raw) << (4 * j);
+    c[j] = clip4(raw);
+  }
+  // x: new_q = s1_q * fx + s0_q * (1 - fx); y: m_z = new_z * fy + new_{2+z} * (1 - fy);
+  // z: v = m_0 * fz + m_1 * (1 - fz)   (src/grid_functions.py:34-42)
+  const float ofx = 1.f - t.fx, ofy = 1.f - t.fy, ofz = 1.f - t.fz;
+  float4 q[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q[i].x = (c[i].x * t.fx) + (c[4 + i].x * ofx);
+    q[i].y = (c[i].y * t.fx) + (c[4 + i].y * ofx);
+    q[i].z = (c[i].z * t.fx) + (c[4 + i].z * ofx);
+    q[i].w = (c[i].w * t.fx) + (c[4 + i].w * ofx);
+  }
+  float4 m[2];
+#pragma unroll
+  for (int z = 0; z < 2; ++z) {
+    m[z].x = (q[z].x * t.fy) + (q[2 + z].x * ofy);
+    m[z].y = (q[z].y * t.fy) + (q[2 + z].y * ofy);
+    m[z].z = (q[z].z * t.fy) + (q[2 + z].z * ofy);
+    m[z].w = (q[z].w * t.fy) + (q[2 + z].w * ofy);
+  }
+  s.v.x = (m[0].x * t.fz) + (m[1].x * ofz);
+  s.v.y = (m[0].y * t.fz) + (m[1].y * ofz);
+  s.v.z = (m[0].z * t.fz) + (m[1].z * ofz);
+  s.v.w = (m[0].w * t.fz) + (m[1].w * ofz);
+  s.keep = true;
+}
+
+template <int DT, int INTERP>
+__device__ __forceinline__ void eval_sample(const KParams& P, const Ray& R, int k, bool active,
+                                            Sample& s) {
+  if constexpr (INTERP == PXL_INTERP_NEAREST)
+    eval_nearest<DT>(P, R, k, active, s);
+  else
+    eval_trilinear<DT>(P, R, k, active, s);
+}
+
+// ----------------------------------------------------------------------------- scatter
+
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+__device__ __forceinline__ void scatter_cell(float* grad, uint32_t cell, uint32_t clip, float g0,
+                                             float g1, float g2, float g3) {
+  float* p = grad + (size_t)cell * 4u;
+  if ((clip & 1u) && g0 != 0.f) atomic_add_f32(p + 0, g0);
+  if ((clip & 2u) && g1 != 0.f) atomic_add_f32(p + 1, g1);
+  if ((clip & 4u) && g2 != 0.f) atomic_add_f32(p + 2, g2);
+  if ((clip & 8u) && g3 != 0.f) atomic_add_f32(p + 3, g3);
+}
+
+// d loss / d (clipped sample value) = gv; route it to the grid cell(s) the sample read.
+template <int INTERP>
+__device__ __forceinline__ void scatter_sample(const KParams& P, const Ray& R, int k,
+                                               uint32_t clip, const float4& gv, float* grad) {
+  float nx, ny, nz;
+  sample_index(P, R, k, nx, ny, nz);
+  if constexpr (INTERP == PXL_INTERP_NEAREST) {
+    uint32_t cell;
+    nearest_cell(P, nx, ny, nz, cell);
+    scatter_cell(grad, cell, clip, gv.x, gv.y, gv.z, gv.w);
+  } else {
+    TriCell t;
+    trilinear_cell(P, nx, ny, nz, t);
+    const float ofx = 1.f - t.fx, ofy = 1.f - t.fy, ofz = 1.f - t.fz;
+#pragma unroll
+    for (int bz = 0; bz < 2; ++bz) {
+      const float wz = bz == 0 ? t.fz : ofz;
+      const float4 gz = make_float4(gv.x * wz, gv.y * wz, gv.z * wz, gv.w * wz);
+#pragma unroll
+      for (int by = 0; by < 2; ++by) {
+        const float wy = by == 0 ? t.fy : ofy;
+        const float4 gy = make_float4(gz.x * wy, gz.y * wy, gz.z * wy, gz.w * wy);
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx) {
+          const float wx = bx == 0 ? t.fx : ofx;
+          const int j = 4 * bx + 2 * by + bz;
+          scatter_cell(grad, t.xs[bx] + t.ys[by] + t.zs[bz], clip >> (4 * j), gy.x * wx,
+                       gy.y * wx, gy.z * wx, gy.w * wx);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace pxl

@@ -1,0 +1,601 @@
+// pxl_render.hip -- fused render / backward / training-step kernels for gfx950 and the
+// C ABI declared in include/pxl.h.
+//
+// Work mapping: one 64-lane wavefront per ray, four rays per 256-thread block.  A ray's
+// samples are processed in chunks of 64 (lane l <-> sample 64c + l + 1):
+//   forward   transmittance T_i = T_chunk * exclusive wave product of (1 - alpha)
+//   backward  suffix colour S_i by a wave suffix scan of the affine maps
+//             q -> alpha_i (c_i, 1) + (1 - alpha_i) q, so dC/dalpha_i = T_i (c_i - S_i)
+//             needs no division by (1 - alpha_i) and stays exact at alpha = 1.
+// Gradients reach the grid through float atomics (global_atomic_add_f32).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "pxl.h"
+#include "pxl_device.h"
+
+namespace pxl {
+
+// ----------------------------------------------------------------------------- kernels
+
+__device__ __forceinline__ int64_t wave_ray(const KParams& P) {
+  return P.ray_offset + (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+}
+
+// Forward: out_rgba (n,4); aux (n, 1 + nch): [processed-sample count, T at each chunk start].
+template <int DT, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restrict__ out,
+                                                        float* __restrict__ aux) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  Ray R;
+  setup_ray(P, ray, R);
+  const int S = P.n_samples, nch = (S + 63) >> 6;
+  float* ax = aux ? aux + ray * P.aux_stride : nullptr;
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  int nproc = S;
+  for (int c = 0; c < nch; ++c) {
+    if (ax && lane == 0) ax[1 + c] = T;
+    const int k = (c << 6) + lane + 1;
+    Sample s;
+    eval_sample<DT, INTERP>(P, R, k, k <= S, s);
+    const float a = s.v.w;
+    const float incl = wave_prefix_mul(1.f - a);
+    const float Ti = T * wave_shift_up(incl, 1.f);
+    bool proc = s.keep;
+    bool stop = false;
+    if (P.t_stop > 0.f) {
+      const uint64_t dead = __ballot(k <= S && Ti < P.t_stop);
+      if (dead) {
+        const int first = __builtin_ctzll(dead);
+        nproc = (c << 6) + first;
+        proc = proc && lane < first;
+        stop = true;
+      }
+    }
+    if (proc) {
+      const float w = a * Ti;
+      cr += s.v.x * w;
+      cg += s.v.y * w;
+      cb += s.v.z * w;
+      ca += w;
+    }
+    if (stop) break;
+    T = T * readlane(incl, 63);
+  }
+  cr = wave_sum(cr);
+  cg = wave_sum(cg);
+  cb = wave_sum(cb);
+  ca = wave_sum(ca);
+  if (lane == 0) {
+    reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
+    if (ax) ax[0] = __int_as_float(nproc);
+  }
+}
+
+// One chunk of the reverse sweep: given the processed samples' clipped values, their
+// transmittance and Qn (the suffix value just behind the chunk), emit dL/dvalue per
+// sample, scatter it, and return the suffix value in front of the chunk.
+template <int INTERP>
+__device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, int lane, int k,
+                                              bool keep, const float4& v, uint32_t clip,
+                                              float Ti, const float4& g, float4& Qn,
+                                              float* grad) {
+  const float a = keep ? v.w : 0.f;
+  Affine4 f;
+  f.a0 = a * v.x;
+  f.a1 = a * v.y;
+  f.a2 = a * v.z;
+  f.a3 = a;
+  f.b = 1.f - a;
+  if (lane == 63) {  // fold the tail: constant map q -> Q of the first sample after the chunk
+    f.a0 = f.a0 + f.b * Qn.x;
+    f.a1 = f.a1 + f.b * Qn.y;
+    f.a2 = f.a2 + f.b * Qn.z;
+    f.a3 = f.a3 + f.b * Qn.w;
+    f.b = 0.f;
+  }
+  wave_suffix_affine(f, lane);  // f.a = Q_i, the colour seen just in front of sample i
+  const float s0 = wave_shift_down(f.a0, Qn.x), s1 = wave_shift_down(f.a1, Qn.y);
+  const float s2 = wave_shift_down(f.a2, Qn.z), s3 = wave_shift_down(f.a3, Qn.w);
+  if (keep) {
+    const float w = a * Ti;
+    float4 gv;
+    gv.x = w * g.x;
+    gv.y = w * g.y;
+    gv.z = w * g.z;
+    gv.w = Ti * (g.x * (v.x - s0) + g.y * (v.y - s1) + g.z * (v.z - s2) + g.w * (1.f - s3));
+    scatter_sample<INTERP>(P, R, k, clip, gv, grad);
+  }
+  Qn = make_float4(readlane(f.a0, 0), readlane(f.a1, 0), readlane(f.a2, 0), readlane(f.a3, 0));
+}
+
+template <int DT, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* __restrict__ grad_out,
+                                                        const float* __restrict__ aux,
+                                                        float* __restrict__ grad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  const float4 g = reinterpret_cast<const float4*>(grad_out)[ray];
+  if (g.x == 0.f && g.y == 0.f && g.z == 0.f && g.w == 0.f) return;
+  Ray R;
+  setup_ray(P, ray, R);
+  const float* ax = aux + ray * P.aux_stride;
+  const int nproc = __float_as_int(ax[0]);
+  const int nch = (nproc + 63) >> 6;
+  float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c = nch - 1; c >= 0; --c) {
+    const int k = (c << 6) + lane + 1;
+    Sample s;
+    eval_sample<DT, INTERP>(P, R, k, k <= nproc, s);
+    const float incl = wave_prefix_mul(1.f - s.v.w);
+    const float Ti = ax[1 + c] * wave_shift_up(incl, 1.f);
+    reverse_chunk<INTERP>(P, R, lane, k, s.keep, s.v, s.clip, Ti, g, Qn, grad);
+  }
+}
+
+// Training step for S <= 64*NCH: the forward sweep keeps every chunk's samples in
+// registers, the loss gradient is formed in place, and the reverse sweep scatters -- one
+// gather of each sample per step.
+template <int DT, int INTERP, int NCH>
+__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* __restrict__ targets,
+                                                        float grad_scale, float* __restrict__ out,
+                                                        float* __restrict__ sq_err,
+                                                        float* __restrict__ grad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  Ray R;
+  setup_ray(P, ray, R);
+  const float4 tgt = targets ? reinterpret_cast<const float4*>(targets)[ray] : camera_target(P, R);
+  const int S = P.n_samples, nch = (S + 63) >> 6;
+  float4 V[NCH];
+  uint32_t CL[NCH];
+  float TI[NCH];
+  uint64_t KEEP[NCH];
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  bool stopped = false;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    V[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    CL[c] = 0u;
+    TI[c] = 0.f;
+    KEEP[c] = 0ull;
+    if (c < nch && !stopped) {
+      const int k = (c << 6) + lane + 1;
+      Sample s;
+      eval_sample<DT, INTERP>(P, R, k, k <= S, s);
+      const float a = s.v.w;
+      const float incl = wave_prefix_mul(1.f - a);
+      const float Ti = T * wave_shift_up(incl, 1.f);
+      bool proc = s.keep;
+      if (P.t_stop > 0.f) {
+        const uint64_t dead = __ballot(k <= S && Ti < P.t_stop);
+        if (dead) {
+          proc = proc && lane < __builtin_ctzll(dead);
+          stopped = true;
+        }
+      }
+      if (proc) {
+        const float w = a * Ti;
+        cr += s.v.x * w;
+        cg += s.v.y * w;
+        cb += s.v.z * w;
+        ca += w;
+      }
+      V[c] = s.v;
+      CL[c] = s.clip;
+      TI[c] = Ti;
+      KEEP[c] = __ballot(proc);
+      T = T * readlane(incl, 63);
+    }
+  }
+  cr = wave_sum(cr);
+  cg = wave_sum(cg);
+  cb = wave_sum(cb);
+  ca = wave_sum(ca);
+  const float d0 = cr - tgt.x, d1 = cg - tgt.y, d2 = cb - tgt.z, d3 = ca - tgt.w;
+  if (lane == 0) {
+    if (out) reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
+    if (sq_err) sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+  }
+  const float4 g = make_float4(grad_scale * d0, grad_scale * d1, grad_scale * d2, grad_scale * d3);
+  float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int c = NCH - 1; c >= 0; --c) {
+    // a chunk with no kept sample is the identity map: Qn passes through unchanged
+    if (c < nch && KEEP[c] != 0ull) {
+      const int k = (c << 6) + lane + 1;
+      const bool keep = (KEEP[c] >> lane) & 1ull;
+      reverse_chunk<INTERP>(P, R, lane, k, keep, V[c], CL[c], TI[c], g, Qn, grad);
+    }
+  }
+}
+
+__global__ void k_camera_rays(KParams P, float* __restrict__ o, float* __restrict__ d,
+                              float* __restrict__ tgt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + P.ray_offset;
+  if (r >= P.n_rays) return;
+  Ray R;
+  camera_ray(P, r, R);
+  o[3 * r] = R.ox;
+  o[3 * r + 1] = R.oy;
+  o[3 * r + 2] = R.oz;
+  d[3 * r] = R.dx;
+  d[3 * r + 1] = R.dy;
+  d[3 * r + 2] = R.dz;
+  if (tgt) reinterpret_cast<float4*>(tgt)[r] = camera_target(P, R);
+}
+
+__global__ void k_dilate(const uint32_t* __restrict__ in, uint32_t X, uint32_t Y, uint32_t Z,
+                         uint32_t* __restrict__ out, int64_t n_words) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  const uint64_t n_cells = (uint64_t)X * Y * Z;
+  uint32_t word = 0u;
+  for (int b = 0; b < 32; ++b) {
+    const uint64_t L = (uint64_t)w * 32u + b;
+    if (L >= n_cells) break;
+    const uint32_t k = (uint32_t)(L % Z), j = (uint32_t)((L / Z) % Y), i = (uint32_t)(L / ((uint64_t)Y * Z));
+    uint32_t any = 0u;
+    for (int c = 0; c < 8 && !any; ++c) {
+      const uint32_t ii = (i + (c >> 2)) % X, jj = (j + ((c >> 1) & 1)) % Y, kk = (k + (c & 1)) % Z;
+      const uint32_t Lc = (ii * Y + jj) * Z + kk;
+      any = (in[Lc >> 5] >> (Lc & 31u)) & 1u;
+    }
+    word |= any << b;
+  }
+  out[w] = word;
+}
+
+__global__ void k_pack_occupancy(const uint8_t* __restrict__ occ, int64_t n_cells,
+                                 uint32_t* __restrict__ out, int64_t n_words) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  uint32_t word = 0u;
+  const int64_t base = w * 32;
+  const int nb = (int)min((int64_t)32, n_cells - base);
+  for (int b = 0; b < nb; ++b) word |= (uint32_t)(occ[base + b] != 0) << b;
+  out[w] = word;
+}
+
+__global__ void k_fill(float4* __restrict__ dst, int64_t n4, float value) {
+  const float4 v = make_float4(value, value, value, value);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+__global__ void k_fill_tail(float* __restrict__ dst, int64_t n, float value) {
+  const int64_t i = threadIdx.x;
+  if (i < n) dst[i] = value;
+}
+
+// Test hook: wave primitives on one wave of inputs (see pxl_debug_wave_ops).
+__global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ out) {
+  const int lane = threadIdx.x;
+  const float x = in[lane];
+  const float incl = wave_prefix_mul(x);
+  out[lane] = incl;
+  out[64 + lane] = wave_shift_up(incl, 1.f);
+  out[128 + lane] = wave_sum(x);
+  Affine4 f{in[64 + lane], in[128 + lane], in[192 + lane], in[256 + lane], in[320 + lane]};
+  wave_suffix_affine(f, lane);
+  out[192 + lane] = f.a0;
+  out[256 + lane] = f.a3;
+  out[320 + lane] = f.b;
+  out[384 + lane] = wave_shift_down(x, -1.f);
+}
+
+}  // namespace pxl
+
+// ============================================================================= host ABI
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    return fail(PXL_E_NO_DEVICE, "%s: no HIP device (%s)", what, hipGetErrorString(e));
+  return fail(PXL_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
+constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // 2^24 blocks of 4 rays
+
+int check_grid(const pxl_grid* g) {
+  if (!g) return fail(PXL_E_INVALID, "grid descriptor is NULL");
+  if (!g->data) return fail(PXL_E_INVALID, "grid data is NULL");
+  if (g->channels != 4) return fail(PXL_E_DTYPE, "grid must have 4 channels, got %d", g->channels);
+  if (g->dtype != PXL_DTYPE_F32 && g->dtype != PXL_DTYPE_BF16 && g->dtype != PXL_DTYPE_F16)
+    return fail(PXL_E_DTYPE, "unsupported grid dtype %d", g->dtype);
+  for (int a = 0; a < 3; ++a)
+    if (g->dims[a] <= 0) return fail(PXL_E_INVALID, "grid dim %d is %lld", a, (long long)g->dims[a]);
+  if (g->dims[0] > kMaxCells || g->dims[1] > kMaxCells || g->dims[2] > kMaxCells ||
+      g->dims[0] * g->dims[1] > kMaxCells || g->dims[0] * g->dims[1] * g->dims[2] > kMaxCells)
+    return fail(PXL_E_SIZE, "grid has more than 2^31-1 cells");
+  return PXL_OK;
+}
+
+int check_march(const pxl_march* m) {
+  if (!m) return fail(PXL_E_INVALID, "march descriptor is NULL");
+  if (m->n_samples < 0 || m->n_samples > (1 << 24))
+    return fail(PXL_E_INVALID, "n_samples %d outside [0, 2^24]", m->n_samples);
+  if (m->interp != PXL_INTERP_NEAREST && m->interp != PXL_INTERP_TRILINEAR)
+    return fail(PXL_E_INVALID, "unknown interp %d", m->interp);
+  if ((m->flags & ~PXL_FLAG_EXACT_DIV) != 0) return fail(PXL_E_INVALID, "unknown flags 0x%x", m->flags);
+  if (!(m->pd > 0.f) || !std::isfinite(m->pd) || !std::isfinite(m->inv_pd))
+    return fail(PXL_E_INVALID, "points_distance must be positive and finite");
+  if (!std::isfinite(m->delta) || !std::isfinite(m->near))
+    return fail(PXL_E_INVALID, "delta/near must be finite");
+  if (!(m->t_stop >= 0.f) || m->t_stop > 1.f) return fail(PXL_E_INVALID, "t_stop must be in [0,1]");
+  return PXL_OK;
+}
+
+int check_rays(const pxl_rays* r, bool need_imgs) {
+  if (!r) return fail(PXL_E_INVALID, "rays descriptor is NULL");
+  if (r->n_rays < 0) return fail(PXL_E_INVALID, "n_rays is negative");
+  if (r->origins) {
+    if (!r->dirs) return fail(PXL_E_INVALID, "origins given without dirs");
+  } else {
+    if (!r->cam_ids || !r->uv || !r->c2w)
+      return fail(PXL_E_INVALID, "camera-form rays need cam_ids, uv and c2w");
+    if (r->n_cams <= 0) return fail(PXL_E_INVALID, "n_cams must be positive");
+  }
+  if (need_imgs) {
+    if (r->origins) return fail(PXL_E_INVALID, "targets from images need camera-form rays");
+    if (!r->imgs || r->img_h <= 0 || r->img_w <= 0)
+      return fail(PXL_E_INVALID, "camera targets need imgs and positive img_h/img_w");
+  }
+  return PXL_OK;
+}
+
+pxl::KParams make_params(const pxl_grid* g, const pxl_march* m, const pxl_rays* r) {
+  pxl::KParams P{};
+  if (g) {
+    P.grid = g->data;
+    P.occ = g->occupancy;
+    P.X = (int)g->dims[0];
+    P.Y = (int)g->dims[1];
+    P.Z = (int)g->dims[2];
+    P.fX = (float)g->dims[0];
+    P.fY = (float)g->dims[1];
+    P.fZ = (float)g->dims[2];
+  }
+  if (m) {
+    P.n_samples = m->n_samples;
+    P.delta = m->delta;
+    P.near = m->near;
+    P.t_stop = m->t_stop;
+    P.pd = m->pd;
+    P.inv_pd = m->inv_pd;
+    P.gx = m->grid_min[0];
+    P.gy = m->grid_min[1];
+    P.gz = m->grid_min[2];
+    P.exact_div = (m->flags & PXL_FLAG_EXACT_DIV) ? 1 : 0;
+    P.aux_stride = (int)pxl_aux_floats(m->n_samples);
+  }
+  P.n_rays = r->n_rays;
+  P.origins = r->origins;
+  P.dirs = r->dirs;
+  P.cam_ids = r->cam_ids;
+  P.uv = r->uv;
+  P.c2w = r->c2w;
+  P.imgs = r->imgs;
+  P.n_cams = r->n_cams;
+  P.img_h = r->img_h;
+  P.img_w = r->img_w;
+  P.theta = r->camera_angle_x;
+  return P;
+}
+
+// Launch `fn(P, blocks)` over all rays in pieces of at most kMaxRaysPerLaunch.
+template <typename F>
+int for_ray_pieces(pxl::KParams P, int64_t rays_per_block, F&& fn, const char* what) {
+  for (int64_t off = 0; off < P.n_rays; off += kMaxRaysPerLaunch) {
+    P.ray_offset = off;
+    const int64_t n = std::min<int64_t>(kMaxRaysPerLaunch, P.n_rays - off);
+    fn(P, (unsigned)((n + rays_per_block - 1) / rays_per_block));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+  }
+  return PXL_OK;
+}
+
+#define PXL_DISPATCH_DT_INTERP(DT, INTERP, BODY)                                     \
+  do {                                                                               \
+    if ((DT) == PXL_DTYPE_F32) {                                                     \
+      constexpr int kDT = PXL_DTYPE_F32;                                             \
+      if ((INTERP) == PXL_INTERP_NEAREST) {                                          \
+        constexpr int kIN = PXL_INTERP_NEAREST;                                      \
+        BODY;                                                                        \
+      } else {                                                                       \
+        constexpr int kIN = PXL_INTERP_TRILINEAR;                                    \
+        BODY;                                                                        \
+      }                                                                              \
+    } else if ((DT) == PXL_DTYPE_BF16) {                                             \
+      constexpr int kDT = PXL_DTYPE_BF16;                                            \
+      if ((INTERP) == PXL_INTERP_NEAREST) {                                          \
+        constexpr int kIN = PXL_INTERP_NEAREST;                                      \
+        BODY;                                                                        \
+      } else {                                                                       \
+        constexpr int kIN = PXL_INTERP_TRILINEAR;                                    \
+        BODY;                                                                        \
+      }                                                                              \
+    } else {                                                                         \
+      constexpr int kDT = PXL_DTYPE_F16;                                             \
+      if ((INTERP) == PXL_INTERP_NEAREST) {                                          \
+        constexpr int kIN = PXL_INTERP_NEAREST;                                      \
+        BODY;                                                                        \
+      } else {                                                                       \
+        constexpr int kIN = PXL_INTERP_TRILINEAR;                                    \
+        BODY;                                                                        \
+      }                                                                              \
+    }                                                                                \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int pxl_abi_version(void) { return PXL_ABI_VERSION; }
+
+const char* pxl_last_error(void) { return g_err.c_str(); }
+
+int64_t pxl_aux_floats(int32_t n_samples) {
+  return 1 + (n_samples > 0 ? ((int64_t)n_samples + 63) / 64 : 0);
+}
+
+int pxl_render_fwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   float* out_rgba, float* aux, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid)) || (rc = check_march(march)) || (rc = check_rays(rays, false))) return rc;
+  if (!out_rgba) return fail(PXL_E_INVALID, "out_rgba is NULL");
+  if (rays->n_rays == 0) return PXL_OK;
+  pxl::KParams P = make_params(grid, march, rays);
+  hipStream_t s = (hipStream_t)stream;
+  PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
+    rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+      hipLaunchKernelGGL((pxl::k_render_fwd<kDT, kIN>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                         out_rgba, aux);
+    }, "pxl_render_fwd");
+  });
+  return rc;
+}
+
+int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   const float* grad_rgba, const float* aux, float* grad_grid, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid)) || (rc = check_march(march)) || (rc = check_rays(rays, false))) return rc;
+  if (!grad_rgba || !aux || !grad_grid) return fail(PXL_E_INVALID, "grad_rgba, aux and grad_grid are required");
+  if (rays->n_rays == 0 || march->n_samples == 0) return PXL_OK;
+  pxl::KParams P = make_params(grid, march, rays);
+  hipStream_t s = (hipStream_t)stream;
+  PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
+    rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+      hipLaunchKernelGGL((pxl::k_render_bwd<kDT, kIN>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                         grad_rgba, aux, grad_grid);
+    }, "pxl_render_bwd");
+  });
+  return rc;
+}
+
+int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   const float* targets, float grad_scale, float* out_rgba, float* ray_sq_err,
+                   float* grad_grid, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid)) || (rc = check_march(march)) ||
+      (rc = check_rays(rays, targets == nullptr)))
+    return rc;
+  if (!grad_grid) return fail(PXL_E_INVALID, "grad_grid is NULL");
+  if (!std::isfinite(grad_scale)) return fail(PXL_E_INVALID, "grad_scale must be finite");
+  if (march->n_samples > 512)
+    return fail(PXL_E_INVALID, "pxl_train_step handles n_samples <= 512 (got %d); use "
+                "pxl_render_fwd + pxl_render_bwd", march->n_samples);
+  if (rays->n_rays == 0) return PXL_OK;
+  pxl::KParams P = make_params(grid, march, rays);
+  hipStream_t s = (hipStream_t)stream;
+  const int S = march->n_samples;
+  PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
+    rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+      if (S <= 64)
+        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 1>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+      else if (S <= 128)
+        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 2>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+      else if (S <= 256)
+        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 4>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+      else
+        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 8>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+    }, "pxl_train_step");
+  });
+  return rc;
+}
+
+int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,
+                    void* stream) {
+  int rc;
+  if ((rc = check_rays(rays, targets != nullptr))) return rc;
+  if (rays->origins) return fail(PXL_E_INVALID, "pxl_camera_rays needs camera-form rays");
+  if (!origins || !dirs) return fail(PXL_E_INVALID, "origins and dirs outputs are required");
+  if (rays->n_rays == 0) return PXL_OK;
+  pxl::KParams P = make_params(nullptr, nullptr, rays);
+  hipStream_t s = (hipStream_t)stream;
+  return for_ray_pieces(P, 256, [&](const pxl::KParams& Q, unsigned nb) {
+    hipLaunchKernelGGL(pxl::k_camera_rays, dim3(nb), dim3(256), 0, s, Q, origins, dirs, targets);
+  }, "pxl_camera_rays");
+}
+
+int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z,
+                         uint32_t* occ_out, void* stream) {
+  if (!occ_in || !occ_out) return fail(PXL_E_INVALID, "occupancy buffers are required");
+  if (occ_in == occ_out) return fail(PXL_E_INVALID, "occ_out must not alias occ_in");
+  if (X <= 0 || Y <= 0 || Z <= 0) return fail(PXL_E_INVALID, "dims must be positive");
+  if (X * Y > kMaxCells || X * Y * Z > kMaxCells) return fail(PXL_E_SIZE, "grid has more than 2^31-1 cells");
+  const int64_t words = (X * Y * Z + 31) / 32;
+  hipLaunchKernelGGL(pxl::k_dilate, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, occ_in, (uint32_t)X, (uint32_t)Y, (uint32_t)Z, occ_out, words);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_occupancy_dilate");
+}
+
+int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream) {
+  if (!occ || !out) return fail(PXL_E_INVALID, "occ and out are required");
+  if (n_cells < 0) return fail(PXL_E_INVALID, "n_cells is negative");
+  if (n_cells > kMaxCells) return fail(PXL_E_SIZE, "more than 2^31-1 cells");
+  if (n_cells == 0) return PXL_OK;
+  const int64_t words = (n_cells + 31) / 32;
+  hipLaunchKernelGGL(pxl::k_pack_occupancy, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, occ, n_cells, out, words);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_pack_occupancy");
+}
+
+int pxl_fill_f32(float* dst, int64_t n, float value, void* stream) {
+  if (n < 0) return fail(PXL_E_INVALID, "n is negative");
+  if (n == 0) return PXL_OK;
+  if (!dst) return fail(PXL_E_INVALID, "dst is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  const uintptr_t addr = (uintptr_t)dst;
+  if (addr % 16 != 0) return fail(PXL_E_INVALID, "dst must be 16-byte aligned");
+  const int64_t n4 = n / 4;
+  if (n4 > 0) {
+    const int64_t want = (n4 + 255) / 256;
+    const unsigned blocks = (unsigned)std::min<int64_t>(want, 256 * 8 * 16);
+    hipLaunchKernelGGL(pxl::k_fill, dim3(blocks), dim3(256), 0, s, reinterpret_cast<float4*>(dst), n4, value);
+  }
+  if (n % 4) hipLaunchKernelGGL(pxl::k_fill_tail, dim3(1), dim3(64), 0, s, dst + n4 * 4, n % 4, value);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_fill_f32");
+}
+
+// Test hook, not part of the render path: runs the wave primitives on one wavefront.
+// in: 6*64 floats (x; affine a0, a1, a2, a3, b), out: 7*64 floats.
+int pxl_debug_wave_ops(const float* in, float* out, void* stream) {
+  if (!in || !out) return fail(PXL_E_INVALID, "in and out are required");
+  hipLaunchKernelGGL(pxl::k_debug_wave, dim3(1), dim3(64), 0, (hipStream_t)stream, in, out);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_debug_wave_ops");
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+"""Fused render path: torch.autograd.Function over the libpxl kernels.
+
+``render_rays`` returns, for a batch of rays, the (R,4) RGBA that the reference training
+chain produces (scripts/train.py:130-153 -- sample placement, normalisation, nearest or
+trilinear gather of grid.clip(0,1) with the out-of-grid mask, compute_alpha_weighted_pixels)
+and differentiates it back to the grid with a hand-written backward kernel instead of
+autograd's per-sample graph.  ``train_step`` fuses render, the mean-MSE gradient
+(scripts/train.py:156) and the backward into one pass.
+
+Extensions (keyword arguments; defaults reproduce the reference): ``near`` offset of the
+sample positions, ``t_stop`` early termination, ``occupancy`` skipping, ``div_mode``.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_DTYPES = {torch.float32: _lib.DTYPE_F32, torch.bfloat16: _lib.DTYPE_BF16,
+           torch.float16: _lib.DTYPE_F16}
+_INTERP = {"nearest": _lib.INTERP_NEAREST, "trilinear": _lib.INTERP_TRILINEAR}
+
+
+@dataclass(frozen=True)
+class MarchConfig:
+    """How samples are placed and looked up (src/ray_sampling.py:161-167, :12-13)."""
+    num_samples: int
+    delta_step: float
+    grid_min: tuple
+    points_distance: float
+    interp: str = "nearest"
+    near: float = 0.0
+    t_stop: float = 0.0
+    div_mode: str = "reciprocal"  # "reciprocal": ATen GPU division; "divide": ATen CPU
+
+    def to_struct(self):
+        m = _lib.March()
+        m.n_samples = int(self.num_samples)
+        if self.interp not in _INTERP:
+            raise ValueError(f"interp must be one of {sorted(_INTERP)}, got {self.interp!r}")
+        m.interp = _INTERP[self.interp]
+        if self.div_mode not in ("reciprocal", "divide"):
+            raise ValueError(f"div_mode must be 'reciprocal' or 'divide', got {self.div_mode!r}")
+        m.flags = _lib.FLAG_EXACT_DIV if self.div_mode == "divide" else 0
+        m.delta = float(np.float32(self.delta_step))
+        m.near = float(np.float32(self.near))
+        m.t_stop = float(self.t_stop)
+        m.pd = float(np.float32(self.points_distance))
+        m.inv_pd = float(np.float32(1.0) / np.float32(self.points_distance))
+        for a in range(3):
+            m.grid_min[a] = float(self.grid_min[a])
+        return m
+
+
+def grid_min_of(grid_indices):
+    """grid_indices.min(0)[0] (src/ray_sampling.py:13) as three float32 values, computed once
+    per grid instead of once per step."""
+    if torch.is_tensor(grid_indices):
+        return tuple(float(x) for x in grid_indices.min(0)[0].float().cpu().tolist())
+    return tuple(float(np.float32(x)) for x in grid_indices)
+
+
+def _grid_struct(grid, occupancy=None):
+    if grid.dim() != 4 or grid.shape[-1] != 4:
+        raise ValueError(f"grid must be (X,Y,Z,4), got {tuple(grid.shape)}")
+    if grid.dtype not in _DTYPES:
+        raise TypeError(f"grid dtype must be float32, bfloat16 or float16, got {grid.dtype}")
+    if not grid.is_cuda:
+        raise ValueError("grid must be a HIP device tensor")
+    if not grid.is_contiguous():
+        raise ValueError("grid must be contiguous")
+    g = _lib.Grid()
+    g.data = grid.data_ptr()
+    for a in range(3):
+        g.dims[a] = grid.shape[a]
+    g.channels = 4
+    g.dtype = _DTYPES[grid.dtype]
+    if occupancy is not None:
+        if occupancy.dtype != torch.int32 or not occupancy.is_cuda or not occupancy.is_contiguous():
+            raise ValueError("occupancy must be a contiguous int32 device tensor of packed bits")
+        need = (grid.shape[0] * grid.shape[1] * grid.shape[2] + 31) // 32
+        if occupancy.numel() < need:
+            raise ValueError(f"occupancy needs {need} words, got {occupancy.numel()}")
+        g.occupancy = occupancy.data_ptr()
+    return g
+
+
+def _rays_struct(origins=None, dirs=None, camera=None):
+    r = _lib.Rays()
+    if origins is not None:
+        for name, t in (("origins", origins), ("dirs", dirs)):
+            if t is None or t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 3:
+                raise ValueError(f"{name} must be (R,3) float32")
+            if not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous device tensor")
+        if origins.shape[0] != dirs.shape[0]:
+            raise ValueError("origins and dirs must have the same number of rays")
+        r.n_rays = origins.shape[0]
+        r.origins = origins.data_ptr()
+        r.dirs = dirs.data_ptr()
+        return r
+    cam = camera
+    r.n_rays = cam.cam_ids.shape[0]
+    r.cam_ids = cam.cam_ids.data_ptr()
+    r.uv = cam.uv.data_ptr()
+    r.c2w = cam.c2w.data_ptr()
+    r.n_cams = cam.c2w.shape[0]
+    r.camera_angle_x = float(np.float32(cam.camera_angle_x))
+    if cam.imgs is not None:
+        r.imgs = cam.imgs.data_ptr()
+        r.img_h = cam.imgs.shape[1]
+        r.img_w = cam.imgs.shape[2]
+    return r
+
+
+@dataclass
+class CameraRays:
+    """Rays in camera form: camera index and uniforms (u,v) per ray, plus the camera stack
+    (src/ray_sampling.py:195-264 with the uniforms drawn by the caller)."""
+    cam_ids: torch.Tensor          # (R,) int32
+    uv: torch.Tensor               # (R,2) float32 in [0,1]
+    c2w: torch.Tensor              # (B,4,4) float32
+    camera_angle_x: float
+    imgs: torch.Tensor = None      # (B,H,W,4) float32 targets
+
+    def validate(self):
+        if self.cam_ids.dtype != torch.int32 or self.cam_ids.dim() != 1:
+            raise ValueError("cam_ids must be (R,) int32")
+        if self.uv.dtype != torch.float32 or tuple(self.uv.shape) != (self.cam_ids.shape[0], 2):
+            raise ValueError("uv must be (R,2) float32")
+        if self.c2w.dtype != torch.float32 or self.c2w.dim() != 3 or tuple(self.c2w.shape[1:]) != (4, 4):
+            raise ValueError("c2w must be (B,4,4) float32")
+        ts = [self.cam_ids, self.uv, self.c2w] + ([self.imgs] if self.imgs is not None else [])
+        for t in ts:
+            if not t.is_cuda or not t.is_contiguous():
+                raise ValueError("camera-form tensors must be contiguous device tensors")
+        if self.imgs is not None:
+            if self.imgs.dtype != torch.float32 or self.imgs.dim() != 4 or self.imgs.shape[-1] != 4:
+                raise ValueError("imgs must be (B,H,W,4) float32")
+            if self.imgs.shape[0] != self.c2w.shape[0]:
+                raise ValueError("imgs and c2w must cover the same cameras")
+
+
+class _RenderFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, grid, origins, dirs, cfg, occupancy):
+        L = _lib.lib()
+        R = origins.shape[0]
+        out = torch.empty((R, 4), dtype=torch.float32, device=grid.device)
+        need_aux = ctx.needs_input_grad[0]
+        aux = (torch.empty((R, int(L.pxl_aux_floats(int(cfg.num_samples)))), dtype=torch.float32,
+                           device=grid.device) if need_aux else None)
+        g = _grid_struct(grid, occupancy)
+        m = cfg.to_struct()
+        r = _rays_struct(origins, dirs)
+        _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
+                                    _lib.ptr(out), _lib.ptr(aux), _lib.stream_handle(grid.device)))
+        ctx.cfg = cfg
+        if need_aux:
+            ctx.save_for_backward(grid, origins, dirs, aux, occupancy)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        grid, origins, dirs, aux, occupancy = ctx.saved_tensors
+        L = _lib.lib()
+        grad_out = grad_out.contiguous().float()
+        grad = torch.zeros(grid.shape, dtype=torch.float32, device=grid.device)
+        g = _grid_struct(grid, occupancy)
+        m = ctx.cfg.to_struct()
+        r = _rays_struct(origins, dirs)
+        _lib.check(L.pxl_render_bwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
+                                    _lib.ptr(grad_out), _lib.ptr(aux), _lib.ptr(grad),
+                                    _lib.stream_handle(grid.device)))
+        return grad.to(grid.dtype), None, None, None, None
+
+
+def render_rays(grid, origins, dirs, *, num_samples, delta_step, grid_min, points_distance,
+                interp="nearest", near=0.0, t_stop=0.0, occupancy=None, div_mode="reciprocal"):
+    """(R,4) RGBA of the reference chain for explicit rays, differentiable w.r.t. grid.
+
+    grid: raw (X,Y,Z,4) parameter (float32, bfloat16 or float16; clipped inside);
+    origins/dirs: (R,3) float32.  grid_min: grid_indices.min(0)[0] (see grid_min_of).
+    occupancy: packed int32 bit words (see pxl.h); trilinear reads the floor cell's bit.
+    """
+    cfg = MarchConfig(int(num_samples), float(delta_step), tuple(grid_min), float(points_distance),
+                      interp, float(near), float(t_stop), div_mode)
+    return _RenderFn.apply(grid, origins.contiguous(), dirs.contiguous(), cfg, occupancy)
+
+
+def camera_rays(camera, want_targets=True):
+    """(origins, dirs, targets) for camera-form rays: src/ray_sampling.py:212-264 per ray."""
+    camera.validate()
+    L = _lib.lib()
+    R = camera.cam_ids.shape[0]
+    dev = camera.c2w.device
+    o = torch.empty((R, 3), dtype=torch.float32, device=dev)
+    d = torch.empty((R, 3), dtype=torch.float32, device=dev)
+    t = torch.empty((R, 4), dtype=torch.float32, device=dev) if want_targets else None
+    r = _rays_struct(camera=camera)
+    _lib.check(L.pxl_camera_rays(ctypes.byref(r), _lib.ptr(o), _lib.ptr(d), _lib.ptr(t),
+                                 _lib.stream_handle(dev)))
+    return o, d, t
+
+
+def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, targets=None,
+               n_total_rays=None, occupancy=None, out=None, ray_sq_err=None):
+    """Render + mean-MSE gradient + backward in one kernel pass.
+
+    Accumulates d mse_loss / d grid into grad_grid ((X,Y,Z,4) float32; zero it first for a
+    plain gradient).  Rays are explicit (origins, dirs) or a CameraRays; targets (R,4), or
+    None to gather them from camera.imgs.  n_total_rays: the loss normaliser 4*N of the full
+    batch this call is part of (default: this call's rays).  Returns ray_sq_err (R,) if
+    requested or allocated: loss = ray_sq_err.sum() / (4 * n_total_rays).
+    """
+    L = _lib.lib()
+    if grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != tuple(grid.shape) or not grad_grid.is_contiguous():
+        raise ValueError("grad_grid must be a contiguous float32 tensor shaped like grid")
+    if camera is not None:
+        camera.validate()
+        r = _rays_struct(camera=camera)
+        if targets is None and camera.imgs is None:
+            raise ValueError("targets or camera.imgs is required")
+    else:
+        r = _rays_struct(origins.contiguous(), dirs.contiguous())
+    R = r.n_rays
+    if targets is not None:
+        if targets.dtype != torch.float32 or tuple(targets.shape) != (R, 4) or not targets.is_contiguous():
+            raise ValueError("targets must be contiguous (R,4) float32")
+    n_total = R if n_total_rays is None else int(n_total_rays)
+    scale = 2.0 / (4.0 * n_total)
+    g = _grid_struct(grid, occupancy)
+    m = cfg.to_struct()
+    _lib.check(L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
+                                _lib.ptr(targets), ctypes.c_float(scale), _lib.ptr(out),
+                                _lib.ptr(ray_sq_err), _lib.ptr(grad_grid),
+                                _lib.stream_handle(grid.device)))
+    return ray_sq_err
+
+
+def fill_(t, value=0.0):
+    """In-place fill of a contiguous float32 device tensor with the library's fill kernel."""
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("fill_ needs a contiguous float32 device tensor")
+    _lib.check(_lib.lib().pxl_fill_f32(_lib.ptr(t), t.numel(), ctypes.c_float(value),
+                                       _lib.stream_handle(t.device)))
+    return t
+
+
+def occupancy_dilate(occ_bits, dims):
+    """Cell occupancy for trilinear skipping from point occupancy bits (see pxl.h)."""
+    X, Y, Z = dims
+    out = torch.empty_like(occ_bits)
+    _lib.check(_lib.lib().pxl_occupancy_dilate(_lib.ptr(occ_bits), X, Y, Z, _lib.ptr(out),
+                                               _lib.stream_handle(occ_bits.device)))
+    return out
+
+
+def pack_occupancy(occ):
+    """(X,Y,Z) bool/uint8 device tensor -> packed int32 bit words for the kernels' occupancy
+    argument (bit (L & 31) of word L >> 5, L the linear cell index)."""
+    if not occ.is_cuda or occ.dim() != 3:
+        raise ValueError("occupancy must be an (X,Y,Z) device tensor")
+    src = occ.to(torch.uint8).contiguous()
+    n = src.numel()
+    out = torch.empty((n + 31) // 32, dtype=torch.int32, device=occ.device)
+    _lib.check(_lib.lib().pxl_pack_occupancy(_lib.ptr(src), n, _lib.ptr(out),
+                                             _lib.stream_handle(occ.device)))
+    return out

@@ -1,0 +1,154 @@
+"""C-ABI checks that need no GPU: libpxl loads, exports every symbol include/pxl.h declares,
+its struct layouts match the ctypes mirrors, and argument validation reports errors before
+any HIP call."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from plenoxels_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pxl.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pxl_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
+    assert set(_lib.SIGNATURES) == set(names)
+    assert L.pxl_abi_version() == 1
+
+
+def test_struct_layout_matches_header(tmp_path):
+    probe = tmp_path / "probe.c"
+    fields = {
+        "pxl_grid": ["data", "dims", "channels", "dtype", "occupancy"],
+        "pxl_march": ["n_samples", "interp", "flags", "reserved", "delta", "near", "t_stop", "pd",
+                      "inv_pd", "grid_min"],
+        "pxl_rays": ["n_rays", "origins", "dirs", "cam_ids", "uv", "c2w", "n_cams",
+                     "camera_angle_x", "reserved", "imgs", "img_h", "img_w"],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pxl.h"', "int main(void){"]
+    for s, fs in fields.items():
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for f in fs:
+            lines.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    lines.append("return 0;}")
+    probe.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(probe), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    mirrors = {"pxl_grid": _lib.Grid, "pxl_march": _lib.March, "pxl_rays": _lib.Rays}
+    for s, cls in mirrors.items():
+        assert int(got[s]) == ctypes.sizeof(cls), s
+        for f in fields[s]:
+            assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, f"{s}.{f}"
+
+
+def _valid():
+    g = _lib.Grid()
+    g.data = 0x1000
+    g.dims[0] = g.dims[1] = g.dims[2] = 8
+    g.channels = 4
+    g.dtype = _lib.DTYPE_F32
+    m = _lib.March()
+    m.n_samples = 16
+    m.interp = _lib.INTERP_NEAREST
+    m.delta = 0.1
+    m.pd = 0.2
+    m.inv_pd = 5.0
+    r = _lib.Rays()
+    r.n_rays = 0
+    r.origins = 0x2000
+    r.dirs = 0x3000
+    return g, m, r
+
+
+def _fwd(g, m, r, out=0x4000):
+    return _lib.lib().pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
+                                     ctypes.c_void_p(out), None, None)
+
+
+def test_validation_errors_before_any_hip_call():
+    L = _lib.lib()
+    g, m, r = _valid()
+    assert _fwd(g, m, r) == _lib.PXL_OK  # zero rays: validated, nothing launched
+    assert L.pxl_render_fwd(None, ctypes.byref(m), ctypes.byref(r), None, None, None) == _lib.PXL_E_INVALID
+    assert "grid" in _lib.last_error()
+    g2, m2, r2 = _valid()
+    g2.channels = 3
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_DTYPE
+    g2, m2, r2 = _valid()
+    g2.dtype = 7
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_DTYPE
+    g2, m2, r2 = _valid()
+    g2.dims[0] = g2.dims[1] = g2.dims[2] = 2048  # 2^33 cells
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_SIZE
+    g2, m2, r2 = _valid()
+    g2.dims[2] = 0
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    m2.n_samples = -1
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    m2.interp = 5
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    m2.pd = 0.0
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    m2.flags = 8
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    r2.n_rays = -3
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    r2.origins = None  # camera form without cameras
+    assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    assert _fwd(g2, m2, r2, out=0) == _lib.PXL_E_INVALID
+    g2, m2, r2 = _valid()
+    m2.n_samples = 600
+    r2.n_rays = 4
+    rc = L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), None, ctypes.c_float(1.0),
+                          None, None, ctypes.c_void_p(0x5000), None)
+    assert rc == _lib.PXL_E_INVALID  # targets missing for explicit rays
+    assert L.pxl_fill_f32(None, 4, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
+    assert L.pxl_fill_f32(ctypes.c_void_p(0x1004), 8, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
+    assert L.pxl_occupancy_dilate(ctypes.c_void_p(0x10), 4, 4, 4, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
+    assert L.pxl_aux_floats(256) == 5
+    assert L.pxl_aux_floats(0) == 1
+    assert L.pxl_aux_floats(65) == 3
+
+
+def test_no_device_is_reported_not_aborted():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    g, m, r = _valid()
+    r.n_rays = 4
+    rc = _fwd(g, m, r)
+    assert rc in (_lib.PXL_E_NO_DEVICE, _lib.PXL_E_HIP)
+    assert _lib.last_error()
+
+
+def test_python_wrappers_reject_cpu_tensors():
+    import torch
+    from plenoxels_amd import render_rays
+    grid = torch.zeros(4, 4, 4, 4)
+    o = torch.zeros(2, 3)
+    with pytest.raises(ValueError):
+        render_rays(grid, o, o, num_samples=4, delta_step=0.1, grid_min=(0, 0, 0), points_distance=0.1)

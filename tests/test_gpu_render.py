@@ -1,0 +1,235 @@
+"""GPU parity of the fused kernels against the oracle, on the same seeded inputs.
+
+Bars: rendered RGBA within 1e-5 absolute of the plain PyTorch path; grid gradients within
+1e-4 relative (of the largest element, and per element above 1% of it) of the plain path's
+autograd gradient and of the FP64 C oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cref
+from oracle import plain
+from plenoxels_amd import CameraRays, MarchConfig, _lib, camera_rays, render_rays, train_step
+from util import grad_close, random_cameras, say, small_scene
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _oracle(grid, o, d, tgt, **kw):
+    g = grid.clone().requires_grad_(True)
+    out = plain.render(g, o, d, **kw)
+    ((out - tgt) ** 2).mean().backward()
+    return out.detach(), g.grad
+
+
+def test_wave_primitives():
+    say("wave primitives")
+    g = torch.Generator().manual_seed(0)
+    x = torch.rand(64, generator=g) * 0.5 + 0.75
+    aff = torch.rand(5, 64, generator=g)
+    inp = torch.cat([x, aff.reshape(-1)]).to(DEV)
+    out = torch.empty(7 * 64, device=DEV)
+    _lib.check(_lib.lib().pxl_debug_wave_ops(_lib.ptr(inp), _lib.ptr(out), _lib.stream_handle(DEV)))
+    out = out.cpu().double().reshape(7, 64)
+    xd = x.double()
+    incl = torch.cumprod(xd, 0)
+    assert torch.allclose(out[0], incl, rtol=1e-6)
+    assert torch.allclose(out[1], torch.cat([torch.ones(1, dtype=torch.float64), incl[:-1]]), rtol=1e-6)
+    assert torch.allclose(out[2], xd.sum().expand(64), rtol=1e-6)
+    a = aff.double()
+    A0, A3, B = torch.zeros(64, dtype=torch.float64), torch.zeros(64, dtype=torch.float64), torch.zeros(64, dtype=torch.float64)
+    acc0, acc3, accb = 0.0, 0.0, 1.0  # composition f_l o ... o f_63 as (a, b)
+    for l in range(63, -1, -1):
+        acc0 = a[0, l] + a[4, l] * acc0
+        acc3 = a[3, l] + a[4, l] * acc3
+        accb = a[4, l] * accb
+        A0[l], A3[l], B[l] = acc0, acc3, accb
+    assert torch.allclose(out[3], A0, rtol=1e-5)
+    assert torch.allclose(out[4], A3, rtol=1e-5)
+    assert torch.allclose(out[5], B, rtol=1e-5, atol=1e-30)
+    assert torch.equal(out[6, :63].float(), x[1:])
+    assert out[6, 63] == -1.0
+
+
+def test_torch_gpu_scalar_division_is_reciprocal_multiply():
+    """Pins oracle.plain's div_mode='reciprocal': ATen's GPU tensor / python-float computes
+    x * float32(1/float32(pd))."""
+    say("torch GPU division semantics")
+    g = torch.Generator().manual_seed(1)
+    x = (torch.rand(1 << 20, generator=g) * 8 - 4)
+    for pd in (0.0125, 0.1, 3 / 255, 3 / 127, 0.3):
+        got = (x.to(DEV) / pd).cpu()
+        assert torch.equal(got, x * plain.reciprocal_f32(pd)), pd
+    # informational: does the GPU's 3-norm use the CPU's sequential-FMA order?
+    v = torch.randn(1 << 16, 3, generator=g)
+    cpu = v.norm(dim=1)
+    gpu = v.to(DEV).norm(dim=1).cpu()
+    say("torch GPU vs CPU 3-norm mismatches:", int((cpu != gpu).sum()))
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+@pytest.mark.parametrize("div_mode", ["reciprocal", "divide"])
+def test_render_fwd_bwd_vs_plain(interp, div_mode):
+    say("render fwd/bwd", interp, div_mode)
+    grid, o, d, gmin, pd = small_scene(10)
+    R = o.shape[0]
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(11))
+    kw = dict(num_samples=96, delta_step=0.04, points_distance=pd, interp=interp)
+    want, gwant = _oracle(grid, o, d, tgt, gmin=gmin, div_mode=div_mode, **kw)
+    gd = grid.to(DEV).requires_grad_(True)
+    out = render_rays(gd, o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), div_mode=div_mode, **kw)
+    ((out - tgt.to(DEV)) ** 2).mean().backward()
+    err = (out.detach().cpu() - want).abs().max().item()
+    say("  rgba max abs err", err)
+    assert err <= 1e-5
+    ok, info = grad_close(gd.grad.cpu(), gwant)
+    say("  grad (max err, scale, per-elem rel)", info)
+    assert ok
+    # and against the FP64 C oracle
+    up = (2 * (out.detach().cpu().double() - tgt.double()) / (4 * R)).numpy()
+    rgba64, g64, _ = cref.render(grid.numpy(), o.numpy(), d.numpy(), grad_rgba=up, gmin=gmin.tolist(),
+                                 div_mode=div_mode, **kw)
+    assert np.abs(out.detach().cpu().double().numpy() - rgba64).max() <= 1e-5
+    ok, info = grad_close(gd.grad.cpu(), torch.from_numpy(g64))
+    say("  grad vs fp64 (max err, scale, per-elem rel)", info)
+    assert ok
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_train_step_equals_autograd_pair(interp):
+    say("train_step vs pair", interp)
+    grid, o, d, gmin, pd = small_scene(20)
+    R = o.shape[0]
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(21))
+    kw = dict(num_samples=150, delta_step=0.026, points_distance=pd, interp=interp)
+    want, gwant = _oracle(grid, o, d, tgt, gmin=gmin, div_mode="reciprocal", **kw)
+    cfg = MarchConfig(150, 0.026, tuple(gmin.tolist()), pd, interp)
+    gd = grid.to(DEV)
+    gg = torch.zeros_like(gd)
+    out = torch.empty(R, 4, device=DEV)
+    sq = torch.empty(R, device=DEV)
+    train_step(gd, gg, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), out=out, ray_sq_err=sq)
+    assert (out.cpu() - want).abs().max().item() <= 1e-5
+    loss = sq.sum().item() / (4 * R)
+    assert abs(loss - ((want - tgt) ** 2).mean().item()) <= 1e-6
+    ok, info = grad_close(gg.cpu(), gwant)
+    say("  grad", info)
+    assert ok
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_half_storage_vs_plain_on_rounded_grid(dtype, interp):
+    say("half storage", dtype, interp)
+    grid, o, d, gmin, pd = small_scene(30)
+    R = o.shape[0]
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(31))
+    rounded = grid.to(dtype).float()
+    kw = dict(num_samples=96, delta_step=0.04, points_distance=pd, interp=interp)
+    want, gwant = _oracle(rounded, o, d, tgt, gmin=gmin, div_mode="reciprocal", **kw)
+    cfg = MarchConfig(96, 0.04, tuple(gmin.tolist()), pd, interp)
+    gg = torch.zeros(grid.shape, device=DEV)
+    out = torch.empty(R, 4, device=DEV)
+    train_step(grid.to(dtype).to(DEV), gg, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV),
+               targets=tgt.to(DEV), out=out)
+    assert (out.cpu() - want).abs().max().item() <= 1e-5
+    ok, info = grad_close(gg.cpu(), gwant)
+    say("  grad", info)
+    assert ok
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_occupancy_skip_vs_oracle(interp):
+    say("occupancy", interp)
+    grid, o, d, gmin, pd = small_scene(40)
+    R = o.shape[0]
+    occ = torch.rand(grid.shape[:3], generator=torch.Generator().manual_seed(41)) < 0.4
+    bits = plain.occupancy_to_bits(occ)
+    bits_dev = bits.to(DEV)
+    if interp == "trilinear":
+        from plenoxels_amd import occupancy_dilate
+        cells = occupancy_dilate(bits_dev, grid.shape[:3])
+        want_cells = plain.occupancy_to_bits(plain.dilate_cells(occ))
+        assert torch.equal(cells.cpu(), want_cells)
+        occ_used, bits_dev = plain.dilate_cells(occ), cells
+    else:
+        occ_used = occ
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(42))
+    kw = dict(num_samples=96, delta_step=0.04, points_distance=pd, interp=interp)
+    want, gwant = _oracle(grid, o, d, tgt, gmin=gmin, div_mode="reciprocal", occupancy=occ_used, **kw)
+    gd = grid.to(DEV).requires_grad_(True)
+    out = render_rays(gd, o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), occupancy=bits_dev, **kw)
+    ((out - tgt.to(DEV)) ** 2).mean().backward()
+    assert (out.detach().cpu() - want).abs().max().item() <= 1e-5
+    ok, info = grad_close(gd.grad.cpu(), gwant)
+    say("  grad", info)
+    assert ok
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_early_termination_vs_fp64_oracle(interp):
+    say("t_stop", interp)
+    grid, o, d, gmin, pd = small_scene(50, alpha_scale=0.5, p_opaque=0.0)
+    R = o.shape[0]
+    t_stop = 0.02
+    kw = dict(num_samples=96, delta_step=0.04, points_distance=pd, interp=interp)
+    rgba64, _, nproc = cref.render(grid.numpy(), o.numpy(), d.numpy(), gmin=gmin.tolist(),
+                                   div_mode="reciprocal", t_stop=t_stop, **kw)
+    assert (nproc < 96).mean() > 0.3  # termination actually happens
+    out = render_rays(grid.to(DEV), o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), t_stop=t_stop, **kw)
+    err = np.abs(out.cpu().double().numpy() - rgba64).max(axis=1)
+    # a ray whose transmittance lands within rounding of t_stop may keep one more sample
+    # (contribution <= t_stop); all others agree to 1e-5
+    assert (err <= 1e-5).mean() >= 0.99
+    assert err.max() <= t_stop
+
+
+def test_long_rays_and_saturation_vs_fp64():
+    """S = 600 (the reference's own run, scripts/main.py) through the fwd/bwd pair, dense
+    opacity near and at 1."""
+    say("long rays S=600")
+    grid, o, d, gmin, pd = small_scene(60, X=32, R=256, pd=0.08, alpha_scale=0.9, p_opaque=0.05)
+    R = o.shape[0]
+    kw = dict(num_samples=600, delta_step=0.0075, points_distance=pd, interp="nearest")
+    gd = grid.to(DEV).requires_grad_(True)
+    out = render_rays(gd, o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), **kw)
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(61))
+    ((out - tgt.to(DEV)) ** 2).mean().backward()
+    up = (2 * (out.detach().cpu().double() - tgt.double()) / (4 * R)).numpy()
+    rgba64, g64, _ = cref.render(grid.numpy(), o.numpy(), d.numpy(), grad_rgba=up, gmin=gmin.tolist(),
+                                 div_mode="reciprocal", **kw)
+    assert np.abs(out.detach().cpu().double().numpy() - rgba64).max() <= 1e-5
+    ok, info = grad_close(gd.grad.cpu(), torch.from_numpy(g64))
+    say("  grad", info)
+    assert ok
+
+
+def test_camera_rays_bitwise_vs_plain():
+    say("camera rays")
+    B, H, W, R = 5, 37, 37, 4096
+    c2w = random_cameras(B, 70, y_scale=1.3)
+    g = torch.Generator().manual_seed(71)
+    imgs = torch.rand(B, H, W, 4, generator=g)
+    cam = torch.randint(0, B, (R,), generator=g)
+    uv = torch.rand(R, 2, generator=g)
+    uv[:8] = torch.tensor([[0.0, 0.0], [1.0, 1.0], [0.5, 0.5], [0.5 / H, 0.5 / W],
+                           [1.5 / H, 2.5 / W], [1.0, 0.0], [0.0, 1.0], [0.25, 0.75]])
+    theta = 0.6911112070083618
+    o_w, d_w, t_w = plain.camera_rays(imgs, c2w, theta, cam, uv[:, 0].contiguous(), uv[:, 1].contiguous())
+    cr = CameraRays(cam.to(torch.int32).to(DEV), uv.to(DEV), c2w.to(DEV), theta, imgs.to(DEV))
+    o, d, t = camera_rays(cr)
+    assert torch.equal(o.cpu(), o_w)
+    assert torch.equal(t.cpu(), t_w)
+    mism = int((d.cpu() != d_w).any(dim=1).sum())
+    say("  direction mismatches (bitwise):", mism)
+    assert mism == 0
+
+
+def test_pack_occupancy_matches_oracle_bits():
+    say("pack occupancy")
+    from plenoxels_amd import pack_occupancy
+    occ = torch.rand(13, 17, 19, generator=torch.Generator().manual_seed(80)) < 0.3
+    got = pack_occupancy(occ.to(DEV)).cpu()
+    assert torch.equal(got, plain.occupancy_to_bits(occ))
