@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=2048)
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-accounting", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3's "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -246,6 +249,15 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty(n, device=dev)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
+        return
     S = Setup(args, dev)
     wall, kern_ms, kern_min_ms = timed_fused(S, args.steps, args.warmup)
     ms_per_step = wall * 1e3 / args.steps

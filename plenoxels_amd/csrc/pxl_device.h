@@ -120,6 +120,20 @@ __device__ __forceinline__ void wave_suffix_affine(Affine4& f, int lane) {
   if (row != 3) f = affine_compose(f, t);
 }
 
+// Ordering of LDS traffic between lanes of one wavefront (no workgroup barrier: the four
+// waves of a block work on different rays and may be at different points).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // ----------------------------------------------------------------------------- grid cells
 
 template <int DT>
@@ -313,108 +327,111 @@ __device__ __forceinline__ bool trilinear_cell(const KParams& P, float nx, float
 
 // ----------------------------------------------------------------------------- samples
 
-struct Sample {
-  float4 v;       // clipped (and interpolated) value; zero unless keep
-  uint32_t clip;  // clip-gradient bits: nearest bit c; trilinear bit 4j+c for corner j
-  bool keep;
-};
+// Floor cell of an in-grid trilinear sample (the cell its occupancy bit is read from).
+__device__ __forceinline__ bool trilinear_floor_cell(const KParams& P, float nx, float ny,
+                                                     float nz, uint32_t& cell) {
+  const bool inb = nx >= 0.f && nx < P.fX && ny >= 0.f && ny < P.fY && nz >= 0.f && nz < P.fZ;
+  cell = inb ? ((uint32_t)floorf(nx) * (uint32_t)P.Y + (uint32_t)floorf(ny)) * (uint32_t)P.Z +
+                   (uint32_t)floorf(nz)
+             : 0u;
+  return inb;
+}
 
-template <int DT>
-__device__ __forceinline__ void eval_nearest(const KParams& P, const Ray& R, int k, bool active,
-                                             Sample& s) {
-  s.v = make_float4(0.f, 0.f, 0.f, 0.f);
-  s.clip = 0u;
-  s.keep = false;
-  if (!active) return;
+// True when sample k lies in the grid and is not skipped by occupancy: the samples whose
+// values the reference chain can make nonzero.  Geometry only -- no grid value is read.
+template <int INTERP>
+__device__ __forceinline__ bool classify(const KParams& P, const Ray& R, int k) {
   float nx, ny, nz;
   sample_index(P, R, k, nx, ny, nz);
   uint32_t cell;
-  if (!nearest_cell(P, nx, ny, nz, cell) || !occupied(P.occ, cell)) return;
-  const float4 raw = load_cell<DT>(P.grid, cell);
-  s.v = clip4(raw);
-  s.clip = clip_bits(raw);
-  s.keep = true;
+  const bool inb = INTERP == PXL_INTERP_NEAREST ? nearest_cell(P, nx, ny, nz, cell)
+                                                : trilinear_floor_cell(P, nx, ny, nz, cell);
+  return inb && occupied(P.occ, cell);
 }
 
-// Corner j = 4*bx + 2*by + bz, b = 0 ceil / 1 floor: the order of
-// src/grid_functions.py:238-243, which trilinear_interpolation (:31-42) consumes.
-template <int DT>
-__device__ __forceinline__ void eval_trilinear(const KParams& P, const Ray& R, int k,
-                                               bool active, Sample& s) {
-  s.v = make_float4(0.f, 0.f, 0.f, 0.f);
-  s.clip = 0u;
-  s.keep = false;
-  if (!active) return;
+__device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.f, 1.f); }
+
+// Clipped value and clip-gradient bit of one channel (bit set iff 0 <= raw <= 1).
+__device__ __forceinline__ float clip_ch(float raw, uint32_t& bits, int bit) {
+  const float c = clamp01(raw);
+  bits |= (uint32_t)(c == raw) << bit;
+  return c;
+}
+
+// Value of a kept sample: nearest cell, or trilinear interpolation of the clipped corners
+// (sum of the eight corner weights (wx*wy)*wz times the corner; the reference's nested lerps
+// src/grid_functions.py:34-42 compute the same polynomial).  clip: bit c (nearest) or
+// 4j + c (trilinear corner j = 4*bx + 2*by + bz, b = 0 ceil / 1 floor, the corner order of
+// src/grid_functions.py:238-243).
+template <int DT, int INTERP>
+__device__ __forceinline__ void eval_kept(const KParams& P, const Ray& R, int k, float4& v,
+                                          uint32_t& clip) {
   float nx, ny, nz;
   sample_index(P, R, k, nx, ny, nz);
-  TriCell t;
-  if (!trilinear_cell(P, nx, ny, nz, t)) return;
-  if (!occupied(P.occ, t.xs[1] + t.ys[1] + t.zs[1])) return;
-  float4 c[8];
+  clip = 0u;
+  if constexpr (INTERP == PXL_INTERP_NEAREST) {
+    uint32_t cell;
+    nearest_cell(P, nx, ny, nz, cell);
+    const float4 raw = load_cell<DT>(P.grid, cell);
+    v.x = clip_ch(raw.x, clip, 0);
+    v.y = clip_ch(raw.y, clip, 1);
+    v.z = clip_ch(raw.z, clip, 2);
+    v.w = clip_ch(raw.w, clip, 3);
+  } else {
+    TriCell t;
+    trilinear_cell(P, nx, ny, nz, t);
+    float4 c[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const float4 raw = load_cell<DT>(P.grid, t.xs[j >> 2] + t.ys[(j >> 1) & 1] + t.zs[j & 1]);
-    s.clip |= clip_bits(raw) << (4 * j);
-    c[j] = clip4(raw);
-  }
-  // x: new_q = s1_q * fx + s0_q * (1 - fx); y: m_z = new_z * fy + new_{2+z} * (1 - fy);
-  // z: v = m_0 * fz + m_1 * (1 - fz)   (src/grid_functions.py:34-42)
-  const float ofx = 1.f - t.fx, ofy = 1.f - t.fy, ofz = 1.f - t.fz;
-  float4 q[4];
+    for (int j = 0; j < 8; ++j)
+      c[j] = load_cell<DT>(P.grid, t.xs[j >> 2] + t.ys[(j >> 1) & 1] + t.zs[j & 1]);
+    const float ofx = 1.f - t.fx, ofy = 1.f - t.fy, ofz = 1.f - t.fz;
+    v = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    q[i].x = (c[i].x * t.fx) + (c[4 + i].x * ofx);
-    q[i].y = (c[i].y * t.fx) + (c[4 + i].y * ofx);
-    q[i].z = (c[i].z * t.fx) + (c[4 + i].z * ofx);
-    q[i].w = (c[i].w * t.fx) + (c[4 + i].w * ofx);
+    for (int j = 0; j < 8; ++j) {
+      const float w = ((j & 4 ? ofx : t.fx) * (j & 2 ? ofy : t.fy)) * (j & 1 ? ofz : t.fz);
+      v.x = fmaf(w, clip_ch(c[j].x, clip, 4 * j + 0), v.x);
+      v.y = fmaf(w, clip_ch(c[j].y, clip, 4 * j + 1), v.y);
+      v.z = fmaf(w, clip_ch(c[j].z, clip, 4 * j + 2), v.z);
+      v.w = fmaf(w, clip_ch(c[j].w, clip, 4 * j + 3), v.w);
+    }
   }
-  float4 m[2];
-#pragma unroll
-  for (int z = 0; z < 2; ++z) {
-    m[z].x = (q[z].x * t.fy) + (q[2 + z].x * ofy);
-    m[z].y = (q[z].y * t.fy) + (q[2 + z].y * ofy);
-    m[z].z = (q[z].z * t.fy) + (q[2 + z].z * ofy);
-    m[z].w = (q[z].w * t.fy) + (q[2 + z].w * ofy);
-  }
-  s.v.x = (m[0].x * t.fz) + (m[1].x * ofz);
-  s.v.y = (m[0].y * t.fz) + (m[1].y * ofz);
-  s.v.z = (m[0].z * t.fz) + (m[1].z * ofz);
-  s.v.w = (m[0].w * t.fz) + (m[1].w * ofz);
-  s.keep = true;
 }
 
-template <int DT, int INTERP>
-__device__ __forceinline__ void eval_sample(const KParams& P, const Ray& R, int k, bool active,
-                                            Sample& s) {
-  if constexpr (INTERP == PXL_INTERP_NEAREST)
-    eval_nearest<DT>(P, R, k, active, s);
-  else
-    eval_trilinear<DT>(P, R, k, active, s);
+// Append the kept samples among chunk k = 64c + lane + 1 (k <= S) to `list` in sample order;
+// returns the new length.  Every lane must be active.
+template <int INTERP>
+__device__ __forceinline__ int append_kept(const KParams& P, const Ray& R, int k, int S,
+                                           int* list, int n) {
+  const bool keep = k <= S && classify<INTERP>(P, R, k);
+  const uint64_t km = __ballot(keep);
+  if (keep) list[n + lane_rank(km)] = k;
+  return n + __popcll(km);
 }
 
 // ----------------------------------------------------------------------------- scatter
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
-__device__ __forceinline__ void scatter_cell(float* grad, uint32_t cell, uint32_t clip, float g0,
-                                             float g1, float g2, float g3) {
-  float* p = grad + (size_t)cell * 4u;
-  if ((clip & 1u) && g0 != 0.f) atomic_add_f32(p + 0, g0);
-  if ((clip & 2u) && g1 != 0.f) atomic_add_f32(p + 1, g1);
-  if ((clip & 4u) && g2 != 0.f) atomic_add_f32(p + 2, g2);
-  if ((clip & 8u) && g3 != 0.f) atomic_add_f32(p + 3, g3);
-}
-
-// d loss / d (clipped sample value) = gv; route it to the grid cell(s) the sample read.
 template <int INTERP>
-__device__ __forceinline__ void scatter_sample(const KParams& P, const Ray& R, int k,
-                                               uint32_t clip, const float4& gv, float* grad) {
+struct ScatterItems {
+  static constexpr int kPerSample = INTERP == PXL_INTERP_NEAREST ? 1 : 8;
+  static constexpr int kPerChunk = kPerSample * kWave;
+};
+
+// Per-sample gradient items: the cells sample k read and d loss / d cell value for each
+// (clip-gradient mask applied), written to LDS at slot `base`.
+template <int INTERP>
+__device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int k, uint32_t clip,
+                                            const float4& gv, uint32_t* s_cell, float4* s_val,
+                                            int base) {
   float nx, ny, nz;
   sample_index(P, R, k, nx, ny, nz);
   if constexpr (INTERP == PXL_INTERP_NEAREST) {
     uint32_t cell;
     nearest_cell(P, nx, ny, nz, cell);
-    scatter_cell(grad, cell, clip, gv.x, gv.y, gv.z, gv.w);
+    s_cell[base] = cell;
+    s_val[base] = make_float4((clip & 1u) ? gv.x : 0.f, (clip & 2u) ? gv.y : 0.f,
+                              (clip & 4u) ? gv.z : 0.f, (clip & 8u) ? gv.w : 0.f);
   } else {
     TriCell t;
     trilinear_cell(P, nx, ny, nz, t);
@@ -431,12 +448,39 @@ __device__ __forceinline__ void scatter_sample(const KParams& P, const Ray& R, i
         for (int bx = 0; bx < 2; ++bx) {
           const float wx = bx == 0 ? t.fx : ofx;
           const int j = 4 * bx + 2 * by + bz;
-          scatter_cell(grad, t.xs[bx] + t.ys[by] + t.zs[bz], clip >> (4 * j), gy.x * wx,
-                       gy.y * wx, gy.z * wx, gy.w * wx);
+          const uint32_t cb = clip >> (4 * j);
+          s_cell[base + j] = t.xs[bx] + t.ys[by] + t.zs[bz];
+          s_val[base + j] = make_float4((cb & 1u) ? gy.x * wx : 0.f, (cb & 2u) ? gy.y * wx : 0.f,
+                                        (cb & 4u) ? gy.z * wx : 0.f, (cb & 8u) ? gy.w * wx : 0.f);
         }
       }
     }
   }
+}
+
+// Scatter one chunk's gradients: kept lanes stage their items in this wave's LDS slice
+// (compacted by rank), then the whole wave flushes them four lanes per cell, so one atomic
+// instruction covers 16 cells and each cell's four channels go out as one 16-byte group.
+template <int INTERP>
+__device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, int lane, int k,
+                                              bool keep, uint32_t clip, const float4& gv,
+                                              float* grad, uint32_t* s_cell, float4* s_val) {
+  constexpr int NI = ScatterItems<INTERP>::kPerSample;
+  const uint64_t km = __ballot(keep);
+  if (km == 0ull) return;
+  const int n = __popcll(km) * NI;
+  if (keep) write_items<INTERP>(P, R, k, clip, gv, s_cell, s_val, lane_rank(km) * NI);
+  wave_lds_sync();
+  const int ch = lane & 3;
+  const float* sv = reinterpret_cast<const float*>(s_val);
+  for (int base = 0; base < n; base += kWave / 4) {
+    const int it = base + (lane >> 2);
+    if (it < n) {
+      const float v = sv[4 * it + ch];
+      if (v != 0.f) atomic_add_f32(grad + (size_t)s_cell[it] * 4u + ch, v);
+    }
+  }
+  wave_lds_sync();
 }
 
 }  // namespace pxl

@@ -1,13 +1,19 @@
 // pxl_render.hip -- fused render / backward / training-step kernels for gfx950 and the
 // C ABI declared in include/pxl.h.
 //
-// Work mapping: one 64-lane wavefront per ray, four rays per 256-thread block.  A ray's
-// samples are processed in chunks of 64 (lane l <-> sample 64c + l + 1):
-//   forward   transmittance T_i = T_chunk * exclusive wave product of (1 - alpha)
-//   backward  suffix colour S_i by a wave suffix scan of the affine maps
-//             q -> alpha_i (c_i, 1) + (1 - alpha_i) q, so dC/dalpha_i = T_i (c_i - S_i)
-//             needs no division by (1 - alpha_i) and stays exact at alpha = 1.
-// Gradients reach the grid through float atomics (global_atomic_add_f32).
+// Work mapping: one 64-lane wavefront per ray, four rays per 256-thread block.
+//   classify   every sample k = 64c + lane + 1 is placed and tested (in grid, occupied) in
+//              chunks of 64; kept samples are compacted, in order, into a per-wave LDS list
+//   groups     the expensive work (cell loads, clip, interpolation, compositing, scatter)
+//              runs on groups of 64 consecutive kept samples, so lanes are not left idle
+//              on skipped samples.  Skipped samples have alpha = 0, so the transmittance
+//              T_i = T_group * exclusive wave product of (1 - alpha) over the group's lanes
+//              equals the reference's cumprod over all samples.
+//   backward   suffix colour S_i by a wave suffix scan of the affine maps
+//              q -> alpha_i (c_i, 1) + (1 - alpha_i) q, so dC/dalpha_i = T_i (c_i - S_i)
+//              needs no division by (1 - alpha_i) and stays exact at alpha = 1.
+// Gradients reach the grid through float atomics (global_atomic_add_f32), staged per group
+// in LDS and flushed four lanes per cell.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,47 +32,79 @@ __device__ __forceinline__ int64_t wave_ray(const KParams& P) {
   return P.ray_offset + (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
 }
 
-// Forward: out_rgba (n,4); aux (n, 1 + nch): [processed-sample count, T at each chunk start].
+// aux per ray (pxl_aux_floats(S) floats): [0] n_proc -- samples k <= n_proc may contribute
+// (S unless early termination stopped the ray); [1] number of groups; then per group g:
+// [2+2g] transmittance before the group, [3+2g] sample index of its first member.
+__device__ __forceinline__ void accumulate(bool proc, float Ti, const float4& v, float& cr,
+                                           float& cg, float& cb, float& ca) {
+  if (proc) {
+    const float w = v.w * Ti;
+    cr += v.x * w;
+    cg += v.y * w;
+    cb += v.z * w;
+    ca += w;
+  }
+}
+
+// Early termination within a group: lanes from the first one whose transmittance is below
+// t_stop on are dropped.  Returns true (and the sample bound) when the ray stops here.
+__device__ __forceinline__ bool stop_check(const KParams& P, bool valid, float Ti, int k,
+                                           int lane, bool& proc, int& n_proc) {
+  if (!(P.t_stop > 0.f)) return false;
+  const uint64_t dead = __ballot(valid && Ti < P.t_stop);
+  if (!dead) return false;
+  const int first = __builtin_ctzll(dead);
+  n_proc = __builtin_amdgcn_readlane(k, first) - 1;
+  proc = proc && lane < first;
+  return true;
+}
+
+// Forward: out_rgba (n,4) and, when aux != NULL, the per-group state the backward needs.
+// Kept samples stream through a 128-entry LDS buffer; any S is handled.
 template <int DT, int INTERP>
 __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restrict__ out,
                                                         float* __restrict__ aux) {
+  __shared__ int s_buf[kWavesPerBlock][2 * kWave];
   const int lane = threadIdx.x & 63;
   const int64_t ray = wave_ray(P);
   if (ray >= P.n_rays) return;
+  int* buf = s_buf[threadIdx.x >> 6];
   Ray R;
   setup_ray(P, ray, R);
   const int S = P.n_samples, nch = (S + 63) >> 6;
   float* ax = aux ? aux + ray * P.aux_stride : nullptr;
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
-  int nproc = S;
-  for (int c = 0; c < nch; ++c) {
-    if (ax && lane == 0) ax[1 + c] = T;
-    const int k = (c << 6) + lane + 1;
-    Sample s;
-    eval_sample<DT, INTERP>(P, R, k, k <= S, s);
-    const float a = s.v.w;
-    const float incl = wave_prefix_mul(1.f - a);
-    const float Ti = T * wave_shift_up(incl, 1.f);
-    bool proc = s.keep;
-    bool stop = false;
-    if (P.t_stop > 0.f) {
-      const uint64_t dead = __ballot(k <= S && Ti < P.t_stop);
-      if (dead) {
-        const int first = __builtin_ctzll(dead);
-        nproc = (c << 6) + first;
-        proc = proc && lane < first;
-        stop = true;
+  int n_proc = S, ng = 0, nbuf = 0;
+  bool stop = false;
+  for (int c = 0; c < nch && !stop; ++c) {
+    nbuf = append_kept<INTERP>(P, R, (c << 6) + lane + 1, S, buf, nbuf);
+    const bool last = c == nch - 1;
+    while (!stop && (nbuf >= kWave || (last && nbuf > 0))) {
+      wave_lds_sync();
+      const int m = min(nbuf, kWave);
+      const bool valid = lane < m;
+      const int k = valid ? buf[lane] : 0;
+      if (ax && lane == 0) {
+        ax[2 + 2 * ng] = T;
+        ax[3 + 2 * ng] = __int_as_float(k);
       }
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t clip;
+      if (valid) eval_kept<DT, INTERP>(P, R, k, v, clip);
+      const float incl = wave_prefix_mul(1.f - v.w);
+      const float Ti = T * wave_shift_up(incl, 1.f);
+      bool proc = valid;
+      stop = stop_check(P, valid, Ti, k, lane, proc, n_proc);
+      accumulate(proc, Ti, v, cr, cg, cb, ca);
+      T = T * readlane(incl, 63);
+      ++ng;
+      const int rest = nbuf - m;  // move the unprocessed tail to the front
+      const int carry = lane < rest ? buf[kWave + lane] : 0;
+      wave_lds_sync();
+      if (lane < rest) buf[lane] = carry;
+      nbuf = rest;
+      wave_lds_sync();
     }
-    if (proc) {
-      const float w = a * Ti;
-      cr += s.v.x * w;
-      cg += s.v.y * w;
-      cb += s.v.z * w;
-      ca += w;
-    }
-    if (stop) break;
-    T = T * readlane(incl, 63);
   }
   cr = wave_sum(cr);
   cg = wave_sum(cg);
@@ -74,18 +112,21 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
   ca = wave_sum(ca);
   if (lane == 0) {
     reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
-    if (ax) ax[0] = __int_as_float(nproc);
+    if (ax) {
+      ax[0] = __int_as_float(n_proc);
+      ax[1] = __int_as_float(ng);
+    }
   }
 }
 
-// One chunk of the reverse sweep: given the processed samples' clipped values, their
-// transmittance and Qn (the suffix value just behind the chunk), emit dL/dvalue per
-// sample, scatter it, and return the suffix value in front of the chunk.
+// One group of the reverse sweep: given the group's clipped values, transmittance and Qn
+// (the suffix value just behind the group), emit dL/dvalue per sample, scatter it, and
+// return in Qn the suffix value in front of the group.
 template <int INTERP>
 __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, const float4& v, uint32_t clip,
                                               float Ti, const float4& g, float4& Qn,
-                                              float* grad) {
+                                              float* grad, uint32_t* s_cell, float4* s_val) {
   const float a = keep ? v.w : 0.f;
   Affine4 f;
   f.a0 = a * v.x;
@@ -93,7 +134,7 @@ __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, in
   f.a2 = a * v.z;
   f.a3 = a;
   f.b = 1.f - a;
-  if (lane == 63) {  // fold the tail: constant map q -> Q of the first sample after the chunk
+  if (lane == 63) {  // fold the tail: constant map q -> Q of the first sample after the group
     f.a0 = f.a0 + f.b * Qn.x;
     f.a1 = f.a1 + f.b * Qn.y;
     f.a2 = f.a2 + f.b * Qn.z;
@@ -103,99 +144,134 @@ __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, in
   wave_suffix_affine(f, lane);  // f.a = Q_i, the colour seen just in front of sample i
   const float s0 = wave_shift_down(f.a0, Qn.x), s1 = wave_shift_down(f.a1, Qn.y);
   const float s2 = wave_shift_down(f.a2, Qn.z), s3 = wave_shift_down(f.a3, Qn.w);
+  float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
   if (keep) {
     const float w = a * Ti;
-    float4 gv;
     gv.x = w * g.x;
     gv.y = w * g.y;
     gv.z = w * g.z;
     gv.w = Ti * (g.x * (v.x - s0) + g.y * (v.y - s1) + g.z * (v.z - s2) + g.w * (1.f - s3));
-    scatter_sample<INTERP>(P, R, k, clip, gv, grad);
   }
+  scatter_chunk<INTERP>(P, R, lane, k, keep, clip, gv, grad, s_cell, s_val);
   Qn = make_float4(readlane(f.a0, 0), readlane(f.a1, 0), readlane(f.a2, 0), readlane(f.a3, 0));
 }
+
+// Per-wave LDS of the kernels that scatter: the kept-sample list and, once the list has been
+// read into registers, the gradient staging area of scatter_chunk.
+template <int INTERP, int LIST>
+struct WaveLds {
+  static constexpr int kItems = ScatterItems<INTERP>::kPerChunk;
+  static constexpr int kStage = kItems * 20;
+  static constexpr int kBytes = (LIST * 4 > kStage ? LIST * 4 : kStage);
+  static __device__ __forceinline__ int* list(unsigned char* base) { return reinterpret_cast<int*>(base); }
+  static __device__ __forceinline__ uint32_t* cells(unsigned char* base) {
+    return reinterpret_cast<uint32_t*>(base);
+  }
+  static __device__ __forceinline__ float4* vals(unsigned char* base) {
+    return reinterpret_cast<float4*>(base + kItems * 4);
+  }
+};
 
 template <int DT, int INTERP>
 __global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* __restrict__ grad_out,
                                                         const float* __restrict__ aux,
                                                         float* __restrict__ grad) {
+  using L = WaveLds<INTERP, kWave>;
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
   const int lane = threadIdx.x & 63;
   const int64_t ray = wave_ray(P);
   if (ray >= P.n_rays) return;
   const float4 g = reinterpret_cast<const float4*>(grad_out)[ray];
   if (g.x == 0.f && g.y == 0.f && g.z == 0.f && g.w == 0.f) return;
+  unsigned char* mem = s_mem[threadIdx.x >> 6];
+  int* list = L::list(mem);
   Ray R;
   setup_ray(P, ray, R);
   const float* ax = aux + ray * P.aux_stride;
-  const int nproc = __float_as_int(ax[0]);
-  const int nch = (nproc + 63) >> 6;
+  const int n_proc = __float_as_int(ax[0]);
+  const int ng = __float_as_int(ax[1]);
   float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int c = nch - 1; c >= 0; --c) {
-    const int k = (c << 6) + lane + 1;
-    Sample s;
-    eval_sample<DT, INTERP>(P, R, k, k <= nproc, s);
-    const float incl = wave_prefix_mul(1.f - s.v.w);
-    const float Ti = ax[1 + c] * wave_shift_up(incl, 1.f);
-    reverse_chunk<INTERP>(P, R, lane, k, s.keep, s.v, s.clip, Ti, g, Qn, grad);
+  for (int gi = ng - 1; gi >= 0; --gi) {
+    const float T0 = ax[2 + 2 * gi];
+    const int k0 = __float_as_int(ax[3 + 2 * gi]);
+    int n = 0;  // the group: the first 64 kept samples from k0 on (k <= n_proc)
+    for (int kb = k0; n < kWave && kb <= n_proc; kb += kWave) {
+      const int k = kb + lane;
+      const bool keep = k <= n_proc && classify<INTERP>(P, R, k);
+      const uint64_t km = __ballot(keep);
+      const int r = n + lane_rank(km);
+      if (keep && r < kWave) list[r] = k;
+      n += __popcll(km);
+    }
+    n = min(n, kWave);
+    wave_lds_sync();
+    const bool valid = lane < n;
+    const int k = valid ? list[lane] : 0;
+    wave_lds_sync();
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t clip = 0u;
+    if (valid) eval_kept<DT, INTERP>(P, R, k, v, clip);
+    const float incl = wave_prefix_mul(1.f - v.w);
+    const float Ti = T0 * wave_shift_up(incl, 1.f);
+    reverse_chunk<INTERP>(P, R, lane, k, valid, v, clip, Ti, g, Qn, grad, L::cells(mem), L::vals(mem));
   }
 }
 
-// Training step for S <= 64*NCH: the forward sweep keeps every chunk's samples in
-// registers, the loss gradient is formed in place, and the reverse sweep scatters -- one
-// gather of each sample per step.
-template <int DT, int INTERP, int NCH>
+// Training step for S <= 64*NG: every kept sample's value stays in registers between the
+// forward sweep and the reverse sweep, the loss gradient is formed in place -- one gather of
+// each kept sample per step.
+template <int DT, int INTERP, int NG>
 __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* __restrict__ targets,
                                                         float grad_scale, float* __restrict__ out,
                                                         float* __restrict__ sq_err,
                                                         float* __restrict__ grad) {
+  using L = WaveLds<INTERP, kWave * NG>;
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
   const int lane = threadIdx.x & 63;
   const int64_t ray = wave_ray(P);
   if (ray >= P.n_rays) return;
+  unsigned char* mem = s_mem[threadIdx.x >> 6];
+  int* list = L::list(mem);
   Ray R;
   setup_ray(P, ray, R);
   const float4 tgt = targets ? reinterpret_cast<const float4*>(targets)[ray] : camera_target(P, R);
   const int S = P.n_samples, nch = (S + 63) >> 6;
-  float4 V[NCH];
-  uint32_t CL[NCH];
-  float TI[NCH];
-  uint64_t KEEP[NCH];
+  int nk = 0;
+  for (int c = 0; c < nch; ++c) nk = append_kept<INTERP>(P, R, (c << 6) + lane + 1, S, list, nk);
+  wave_lds_sync();
+  const int ng = (nk + 63) >> 6;
+  float4 V[NG];
+  uint32_t CL[NG];
+  float TI[NG];
+  int KI[NG];
+  uint64_t KEEP[NG];
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  int n_proc = S;
   bool stopped = false;
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    V[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-    CL[c] = 0u;
-    TI[c] = 0.f;
-    KEEP[c] = 0ull;
-    if (c < nch && !stopped) {
-      const int k = (c << 6) + lane + 1;
-      Sample s;
-      eval_sample<DT, INTERP>(P, R, k, k <= S, s);
-      const float a = s.v.w;
-      const float incl = wave_prefix_mul(1.f - a);
+  for (int g = 0; g < NG; ++g) {
+    V[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    CL[g] = 0u;
+    TI[g] = 0.f;
+    KI[g] = 0;
+    KEEP[g] = 0ull;
+    if (g < ng && !stopped) {
+      const int idx = (g << 6) + lane;
+      const bool valid = idx < nk;
+      const int k = valid ? list[idx] : 0;
+      if (valid) eval_kept<DT, INTERP>(P, R, k, V[g], CL[g]);
+      const float incl = wave_prefix_mul(1.f - V[g].w);
       const float Ti = T * wave_shift_up(incl, 1.f);
-      bool proc = s.keep;
-      if (P.t_stop > 0.f) {
-        const uint64_t dead = __ballot(k <= S && Ti < P.t_stop);
-        if (dead) {
-          proc = proc && lane < __builtin_ctzll(dead);
-          stopped = true;
-        }
-      }
-      if (proc) {
-        const float w = a * Ti;
-        cr += s.v.x * w;
-        cg += s.v.y * w;
-        cb += s.v.z * w;
-        ca += w;
-      }
-      V[c] = s.v;
-      CL[c] = s.clip;
-      TI[c] = Ti;
-      KEEP[c] = __ballot(proc);
+      bool proc = valid;
+      stopped = stop_check(P, valid, Ti, k, lane, proc, n_proc);
+      accumulate(proc, Ti, V[g], cr, cg, cb, ca);
+      TI[g] = Ti;
+      KI[g] = k;
+      KEEP[g] = __ballot(proc);
       T = T * readlane(incl, 63);
     }
   }
+  wave_lds_sync();  // the list area becomes the scatter staging area
   cr = wave_sum(cr);
   cg = wave_sum(cg);
   cb = wave_sum(cb);
@@ -205,15 +281,15 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* _
     if (out) reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
     if (sq_err) sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
   }
-  const float4 g = make_float4(grad_scale * d0, grad_scale * d1, grad_scale * d2, grad_scale * d3);
+  const float4 gl = make_float4(grad_scale * d0, grad_scale * d1, grad_scale * d2, grad_scale * d3);
   float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-  for (int c = NCH - 1; c >= 0; --c) {
-    // a chunk with no kept sample is the identity map: Qn passes through unchanged
-    if (c < nch && KEEP[c] != 0ull) {
-      const int k = (c << 6) + lane + 1;
-      const bool keep = (KEEP[c] >> lane) & 1ull;
-      reverse_chunk<INTERP>(P, R, lane, k, keep, V[c], CL[c], TI[c], g, Qn, grad);
+  for (int g = NG - 1; g >= 0; --g) {
+    // a group with no processed sample is the identity map: Qn passes through unchanged
+    if (g < ng && KEEP[g] != 0ull) {
+      const bool keep = (KEEP[g] >> lane) & 1ull;
+      reverse_chunk<INTERP>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, Qn, grad,
+                            L::cells(mem), L::vals(mem));
     }
   }
 }
@@ -460,7 +536,7 @@ int pxl_abi_version(void) { return PXL_ABI_VERSION; }
 const char* pxl_last_error(void) { return g_err.c_str(); }
 
 int64_t pxl_aux_floats(int32_t n_samples) {
-  return 1 + (n_samples > 0 ? ((int64_t)n_samples + 63) / 64 : 0);
+  return 2 + 2 * (n_samples > 0 ? ((int64_t)n_samples + 63) / 64 : 0);
 }
 
 int pxl_render_fwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,

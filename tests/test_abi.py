@@ -129,9 +129,9 @@ def test_validation_errors_before_any_hip_call():
     assert L.pxl_fill_f32(None, 4, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_fill_f32(ctypes.c_void_p(0x1004), 8, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_occupancy_dilate(ctypes.c_void_p(0x10), 4, 4, 4, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
-    assert L.pxl_aux_floats(256) == 5
-    assert L.pxl_aux_floats(0) == 1
-    assert L.pxl_aux_floats(65) == 3
+    assert L.pxl_aux_floats(256) == 10
+    assert L.pxl_aux_floats(0) == 2
+    assert L.pxl_aux_floats(65) == 6
 
 
 def test_no_device_is_reported_not_aborted():

@@ -233,3 +233,42 @@ def test_pack_occupancy_matches_oracle_bits():
     occ = torch.rand(13, 17, 19, generator=torch.Generator().manual_seed(80)) < 0.3
     got = pack_occupancy(occ.to(DEV)).cpu()
     assert torch.equal(got, plain.occupancy_to_bits(occ))
+
+
+@pytest.mark.parametrize("t_stop", [0.0, 1e-4])
+def test_sparse256_subset_train_step_vs_plain(t_stop):
+    """The benchmark workload itself (256^3, ~5% occupancy, trilinear, camera-form rays,
+    targets gathered from resident images) on a 2048-ray subset, fused train step vs the
+    plain chain on the CPU with the same rays."""
+    from plenoxels_amd import occupancy_dilate, pack_occupancy, scenes
+    say("sparse256 subset t_stop", t_stop)
+    w = scenes.make_workload("sparse256", n_rays=1 << 18)
+    n = 2048
+    cam, uv = w.cam_ids[:n].contiguous(), w.uv[:n].contiguous()
+    imgs = torch.rand(w.c2w.shape[0], 800, 800, 4, generator=torch.Generator().manual_seed(3))
+    o, d, tgt = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam.long(), uv[:, 0].contiguous(),
+                                  uv[:, 1].contiguous())
+    occ_cells = plain.dilate_cells(w.occupancy)
+    kw = dict(num_samples=w.num_samples, delta_step=w.delta_step, points_distance=w.points_distance,
+              interp="trilinear", near=w.near)
+    g = w.grid.clone().requires_grad_(True)
+    want = plain.render(g, o, d, gmin=torch.tensor(w.grid_min), div_mode="reciprocal",
+                        occupancy=occ_cells, t_stop=t_stop, **kw)
+    ((want - tgt) ** 2).mean().backward()
+    grid = w.grid.to(DEV)
+    bits = occupancy_dilate(pack_occupancy(w.occupancy.to(DEV)), w.dims)
+    assert torch.equal(bits.cpu(), plain.occupancy_to_bits(occ_cells))
+    cr = CameraRays(cam.to(DEV), uv.to(DEV), w.c2w.to(DEV), w.camera_angle_x, imgs.to(DEV))
+    cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance, "trilinear", w.near, t_stop)
+    gg = torch.zeros_like(grid)
+    out = torch.empty(n, 4, device=DEV)
+    train_step(grid, gg, cfg=cfg, camera=cr, occupancy=bits, out=out)
+    err = (out.cpu() - want.detach()).abs().max(dim=1).values
+    say("  rgba: max err", err.max().item(), "rays > 1e-5:", int((err > 1e-5).sum()))
+    ok, info = grad_close(gg.cpu(), g.grad)
+    say("  grad", info)
+    if t_stop == 0.0:
+        assert err.max().item() <= 1e-5
+        assert ok
+    else:
+        assert (err <= 1e-5).float().mean().item() >= 0.995
