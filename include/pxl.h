@@ -120,16 +120,25 @@ int pxl_render_fwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
 int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
                    const float* grad_rgba, const float* aux, float* grad_grid, void* stream);
 
-/* One training step's render + loss gradient + backward in one pass:
- * out = render(rays); dL/dout = grad_scale * (out - target); accumulates dL/dgrid.
- * targets (n,4) f32, or NULL in camera form with imgs set (targets gathered from imgs as
- * src/ray_sampling.py:238-248 does).  grad_scale = 2/(4*N_total) reproduces mse_loss's
- * mean (scripts/train.py:156).  ray_sq_err (n,) f32 or NULL: sum over channels of
- * (out-target)^2 per ray.  out_rgba may be NULL.
- * Replaces scripts/train.py:130-157 + :180-181 minus the optimiser. */
+/* Loss terms of one training step (scripts/train.py:150-177). */
+typedef struct pxl_loss {
+  const float* targets; /* (n,4) f32, or NULL in camera form with imgs set (targets gathered
+                           from imgs as src/ray_sampling.py:238-248 does) */
+  float grad_scale;     /* d loss / d out = grad_scale * (out - target); 2/(4*N_rays) is
+                           mse_loss's mean (scripts/train.py:156) */
+  float beta_scale;     /* beta / N_samples: weight of the per-sample term
+                           f(a) = log(a + 1e-4) - log(1 - a + 1e-4) averaged over all samples
+                           (scripts/train.py:170-174); 0 disables it */
+  float* ray_sq_err;    /* (n,) or NULL: sum over channels of (out - target)^2 */
+  float* ray_beta;      /* (n,) or NULL: sum of f(a) - f(0) over the ray's processed samples
+                           (every other sample contributes f(0)) */
+} pxl_loss;
+
+/* One training step's render + loss gradient + backward in one pass: out = render(rays);
+ * accumulates d loss / d grid into grad_grid.  out_rgba may be NULL.  n_samples <= 512.
+ * Replaces scripts/train.py:130-177 + :180-181 minus the optimiser and the TV term. */
 int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
-                   const float* targets, float grad_scale, float* out_rgba,
-                   float* ray_sq_err, float* grad_grid, void* stream);
+                   const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream);
 
 /* Ray generation only (src/ray_sampling.py:195-264 on per-ray camera/uniforms):
  * origins (n,3), unit dirs (n,3), targets (n,4) (NULL to skip; needs imgs). */
@@ -147,6 +156,102 @@ int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void*
 
 /* dst[0..n) = value (HBM-rate fill, e.g. the per-step gradient zero-fill). */
 int pxl_fill_f32(float* dst, int64_t n, float value, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Per-function kernels behind the reference's individual call signatures.  Unless stated
+ * otherwise: float32, contiguous, device pointers; N = number of points; grids (X,Y,Z,4).
+ * ------------------------------------------------------------------------------------- */
+
+/* src/ray_sampling.py:161-167: out[(r*S + k-1)] = o + d_r * (delta*k + near), k = 1..S, with
+ * o = origins[r / rays_per_origin] (the reference repeats each camera position over its
+ * rays).  out: (n_rays*S, 3). */
+int pxl_sample_points(const float* origins, int64_t rays_per_origin, const float* dirs,
+                      int64_t n_rays, int32_t n_samples, float delta, float near, float* out,
+                      void* stream);
+
+/* src/ray_sampling.py:13 grid_indices.min(0)[0]: out3 (3 floats, device) = column minima of
+ * coords (n,3). */
+int pxl_coord_min(const float* coords, int64_t n, float* out3, void* stream);
+
+/* src/ray_sampling.py:12-13: out = (p - min3) * inv_pd (or / pd with PXL_FLAG_EXACT_DIV);
+ * min3 is a device pointer (3 floats). */
+int pxl_normalize_points(const float* p, int64_t n, const float* min3, float pd, float inv_pd,
+                         int32_t flags, float* out, void* stream);
+
+/* src/grid_functions.py:47-63 (find_out_of_bound, True = inside): idx (N,3) int64 when
+ * is_float == 0, float32 otherwise; mask (N,) bytes. */
+int pxl_in_bounds(const void* idx, int32_t is_float, int64_t n, const int64_t dims[3],
+                  uint8_t* mask, void* stream);
+
+/* src/grid_functions.py:66-79 (fix_out_of_bounds): idx (N,3) int64 wrapped in place,
+ * result in [0,dim) (Python modulo). */
+int pxl_wrap_indices(int64_t* idx, int64_t n, const int64_t dims[3], void* stream);
+
+/* src/grid_functions.py:103-114 (get_nearest_voxels): idx = round-half-even(n), mask =
+ * in-bounds, vals = grid[idx wrapped] (every point, as the reference gathers before the
+ * caller applies the mask).  vals (N,4), mask (N,) bytes.  grid: f32. */
+int pxl_nearest_gather(const float* n, int64_t N, const pxl_grid* grid, float* vals,
+                       uint8_t* mask, void* stream);
+/* backward of the gather: grad_grid[idx wrapped] += grad_vals (N,4) */
+int pxl_nearest_scatter(const float* n, int64_t N, const pxl_grid* grid, const float* grad_vals,
+                        float* grad_grid, void* stream);
+
+/* src/grid_functions.py:220-246 (get_grid_points_indices): corners (N,8,3) int64,
+ * x{ceil,floor} x y{ceil,floor} x z{ceil,floor}. */
+int pxl_corner_indices(const float* n, int64_t N, int64_t* corners, void* stream);
+
+/* src/grid_functions.py:154-170 (collect_cell_information_via_indices): out (N*8,4) = grid at
+ * the wrapped corners, mask (N,) = float in-bounds test.  grid: f32. */
+int pxl_collect_cells(const float* n, int64_t N, const pxl_grid* grid, float* out, uint8_t* mask,
+                      void* stream);
+int pxl_collect_cells_bwd(const float* n, int64_t N, const pxl_grid* grid, const float* grad_out,
+                          float* grad_grid, void* stream);
+
+/* src/grid_functions.py:7-44 (trilinear_interpolation): out (N,4) from corners (N,8,3) int64
+ * (pre-wrapped, as the reference requires; out-of-range corners are wrapped here instead of
+ * raising) with signed frac weights, in the reference's lerp order.  grid: f32. */
+int pxl_trilinear(const float* n, const int64_t* corners, int64_t N, const pxl_grid* grid,
+                  float* out, void* stream);
+int pxl_trilinear_bwd(const float* n, const int64_t* corners, int64_t N, const pxl_grid* grid,
+                      const float* grad_out, float* grad_grid, void* stream);
+
+/* src/ray_sampling.py:172-192 (compute_alpha_weighted_pixels): samples (R,S,4) -> out (R,4);
+ * backward writes grad_samples (R,S,4) (overwritten, not accumulated).  S <= 65536. */
+int pxl_composite_fwd(const float* samples, int64_t R, int32_t S, float* out, void* stream);
+int pxl_composite_bwd(const float* samples, int64_t R, int32_t S, const float* grad_out,
+                      float* grad_samples, void* stream);
+
+/* src/grid_functions.py:198-212 (generate_grid's coordinates): coords (X,Y,Z,3) f32 with
+ * cell (i,j,k) at ((i,j,k) - ceil(S/2) + 1) * pd; mesh (X,Y,Z,3) int64 or NULL. */
+int pxl_grid_coords(int64_t X, int64_t Y, int64_t Z, float pd, float* coords, int64_t* mesh,
+                    void* stream);
+
+/* src/grid_functions.py:173-181 (average_pool3d_grid): avg_pool3d with kernel k, stride s, no
+ * padding over the (X,Y,Z,4) layout directly; out (Xo,Yo,Zo,4), Xo = (X-k)/s + 1. */
+int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s,
+                   float* out, void* stream);
+/* backward: grad_grid (X,Y,Z,4) += sum over windows containing a cell of grad_out / k^3 */
+int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
+                       int32_t s, float* grad_grid, void* stream);
+
+/* scripts/train.py:44-65 (tv_loss): sum_sq (1 double, device, accumulated) += sum over the
+ * three axes of squared neighbour differences of grid (X,Y,Z,4). */
+int pxl_tv_sumsq(const float* grid, int64_t X, int64_t Y, int64_t Z, double* sum_sq, void* stream);
+/* d(weight * sqrt(sum_sq))/dgrid accumulated into grad_grid; sum_sq read on the device. */
+int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double* sum_sq,
+               float weight, float* grad_grid, void* stream);
+
+/* torch.optim.Adam step (scripts/train.py:89,182; defaults betas (0.9,0.999), eps 1e-8, no
+ * weight decay) fused with the |grad| accumulation of scripts/train.py:184:
+ *   m = lerp(m, g, 1-b1); v = b2 v + (1-b2) g^2;
+ *   p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps);  grad_abs += |g|
+ * The scalar coefficients are formed in double and rounded to float32, as torch forms them
+ * in Python.  param/exp_avg/exp_avg_sq/grad FP32 (n).  grad_abs may be NULL.  half_out (n) of
+ * half_dtype (PXL_DTYPE_BF16 / PXL_DTYPE_F16, round to nearest even) receives the updated
+ * parameter for a reduced-precision render copy, or NULL. */
+int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                  float* grad_abs, int64_t n, double lr, double beta1, double beta2, double eps,
+                  int64_t step, void* half_out, int32_t half_dtype, void* stream);
 
 /* Host-side traffic accounting (no counterpart in the reference; ALL POINTERS ARE HOST
  * POINTERS, including grid->occupancy, and grid->data is not read): counts[r] = number of

@@ -54,7 +54,7 @@ def grid_min(grid_indices):
 def reciprocal_f32(points_distance):
     """float32(1 / float32(pd)): the multiplier ATen's GPU true-division uses for a scalar
     divisor (ATen BinaryDivTrueKernel.cu, outside the reference).  Pinned on the GPU by
-    tests/test_gpu_semantics.py."""
+    tests/test_gpu_render.py::test_torch_gpu_scalar_division_is_reciprocal_multiply."""
     return float(np.float32(1.0) / np.float32(points_distance))
 
 
@@ -310,3 +310,80 @@ def dilate_cells(occ):
             for c in (0, 1):
                 out |= torch.roll(occ, shifts=(-a, -b, -c), dims=(0, 1, 2))
     return out
+
+
+# --------------------------------------------------------------------------- training loop
+
+RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
+
+
+def average_pool3d_grid(tensor, receptive_field_size=3, stride=None):
+    """src/grid_functions.py:173-181."""
+    inp = tensor.permute(3, 0, 1, 2).unsqueeze(0)
+    k = (receptive_field_size,) * 3
+    out = F.avg_pool3d(inp, k, stride=stride)
+    return out.squeeze().permute(1, 2, 3, 0)
+
+
+def tv_loss(input_tensor):
+    """scripts/train.py:44-65."""
+    gx = (input_tensor[:, :-1, :, :] - input_tensor[:, 1:, :, :]).pow(2)
+    gy = (input_tensor[:, :, :-1, :] - input_tensor[:, :, 1:, :]).pow(2)
+    gz = (input_tensor[:-1, :, :, :] - input_tensor[1:, :, :, :]).pow(2)
+    return torch.sqrt(gx.sum() + gy.sum() + gz.sum())
+
+
+def reference_fit(imgs, c2w, camera_angle_x, *, gridsize, points_distance, num_samples, delta_step, lr, tv,
+                  beta, uniforms, progressive_growing=True, div_mode="divide", start_step=0, init_grid=None):
+    """scripts/train.py:79-184 op for op on given per-step uniforms (list of (B,R,2)).
+
+    Returns (grid, grid_grad_abs_sum, per-step loss dicts).  Rays: camera-major, R per camera.
+    """
+    dev = imgs.device
+    grid_indices, grid_full, _, grid_grid_original = generate_grid(*gridsize, points_distance=points_distance,
+                                                                   device=dev)
+    if init_grid is not None:  # (the reference starts from zeros, src/grid_functions.py:214)
+        with torch.no_grad():
+            grid_full.copy_(init_grid)
+    grad_abs = torch.zeros_like(grid_full)
+    opt = torch.optim.Adam([grid_full], lr=lr)
+    hist = []
+    for i, uv in enumerate(uniforms, start=start_step):
+        k = RECEPTIVE_FIELDS[i // 5] if i // 5 < len(RECEPTIVE_FIELDS) else 0
+        if k > 1 and progressive_growing:
+            stride = max(1, k // 4)
+            start = int(k / 2)
+            gg = grid_grid_original[start::stride, start::stride, start::stride]
+            gi = gg.reshape(-1, 3)
+            pd = points_distance * stride
+            cells = average_pool3d_grid(grid_full, receptive_field_size=k, stride=stride)
+        else:
+            cells = grid_full
+            gi = grid_grid_original.reshape(-1, 3)
+            pd = points_distance
+        B, R = uv.shape[0], uv.shape[1]
+        d, pix = generate_rays_batched(imgs, R, c2w, camera_angle_x, uv)
+        o = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
+        p = sample_positions(o, d, num_samples, delta_step)
+        n = normalize_samples(p, grid_min(gi), pd, div_mode)
+        nearest, mask = nearest_values(n, cells)
+        nearest = nearest.reshape(B, R, num_samples, 4)
+        col = compute_alpha_weighted_pixels(nearest).reshape(B * R, 4)
+        closs = F.mse_loss(col, pix)
+        loss = closs
+        rec = {"closs": float(closs.detach())}
+        if tv > 0 and k < 19:
+            t = tv * tv_loss(cells)
+            loss = loss + t
+            rec["tvloss"] = float(t.detach())
+        if beta > 0:
+            a = nearest[..., -1]
+            b = beta * (torch.log(a + 1e-4) - torch.log(1 - a + 1e-4)).mean()
+            loss = loss + b
+            rec["betaloss"] = float(b.detach())
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        grad_abs += torch.abs(grid_full.grad)
+        hist.append(rec)
+    return grid_full.detach(), grad_abs, hist

@@ -47,6 +47,11 @@ class Rays(ctypes.Structure):
                 ("img_h", ctypes.c_int64), ("img_w", ctypes.c_int64)]
 
 
+class Loss(ctypes.Structure):
+    _fields_ = [("targets", c_void_p), ("grad_scale", ctypes.c_float),
+                ("beta_scale", ctypes.c_float), ("ray_sq_err", c_void_p), ("ray_beta", c_void_p)]
+
+
 _GP = ctypes.POINTER(Grid)
 _MP = ctypes.POINTER(March)
 _RP = ctypes.POINTER(Rays)
@@ -58,13 +63,48 @@ SIGNATURES = {
     "pxl_aux_floats": (ctypes.c_int64, [ctypes.c_int32]),
     "pxl_render_fwd": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, c_void_p, c_void_p]),
     "pxl_render_bwd": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "pxl_train_step": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, ctypes.c_float, c_void_p,
-                                      c_void_p, c_void_p, c_void_p]),
+    "pxl_train_step": (ctypes.c_int, [_GP, _MP, _RP, ctypes.POINTER(Loss), c_void_p, c_void_p,
+                                      c_void_p]),
     "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, c_void_p, c_void_p]),
     "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
     "pxl_fill_f32": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p]),
+    "pxl_sample_points": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64,
+                                         ctypes.c_int32, ctypes.c_float, ctypes.c_float, c_void_p,
+                                         c_void_p]),
+    "pxl_coord_min": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_normalize_points": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_int32, c_void_p, c_void_p]),
+    "pxl_in_bounds": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                     ctypes.POINTER(ctypes.c_int64), c_void_p, c_void_p]),
+    "pxl_wrap_indices": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                        c_void_p]),
+    "pxl_nearest_gather": (ctypes.c_int, [c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p, c_void_p]),
+    "pxl_nearest_scatter": (ctypes.c_int, [c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p, c_void_p]),
+    "pxl_corner_indices": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_collect_cells": (ctypes.c_int, [c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p, c_void_p]),
+    "pxl_collect_cells_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p,
+                                             c_void_p]),
+    "pxl_trilinear": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p]),
+    "pxl_trilinear_bwd": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, _GP, c_void_p, c_void_p,
+                                         c_void_p]),
+    "pxl_composite_fwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int32, c_void_p, c_void_p]),
+    "pxl_composite_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int32, c_void_p, c_void_p,
+                                         c_void_p]),
+    "pxl_grid_coords": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                       c_void_p, c_void_p, c_void_p]),
+    "pxl_avg_pool3d": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+    "pxl_avg_pool3d_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+    "pxl_tv_sumsq": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,
+                                    c_void_p]),
+    "pxl_tv_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,
+                                  ctypes.c_float, c_void_p, c_void_p]),
+    "pxl_adam_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int64, c_void_p, ctypes.c_int32, c_void_p]),
     "pxl_count_cells_host": (ctypes.c_int, [_GP, _MP, c_void_p, c_void_p, ctypes.c_int64,
                                             c_void_p, c_void_p, ctypes.c_int32]),
     "pxl_debug_wave_ops": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
@@ -117,3 +157,8 @@ def ptr(t):
 def stream_handle(device=None):
     """hipStream_t of torch's current stream on `device`."""
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def dims_arg(dims):
+    """int64[3] argument for the entry points that take grid dims."""
+    return (ctypes.c_int64 * 3)(*[int(x) for x in dims])

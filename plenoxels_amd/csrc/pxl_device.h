@@ -162,16 +162,10 @@ __device__ __forceinline__ float4 load_cell<PXL_DTYPE_F16>(const void* grid, uin
                      half_bits_to_float(w.y & 0xffffu), half_bits_to_float(w.y >> 16));
 }
 
-__device__ __forceinline__ float clip01(float v) { return fminf(fmaxf(v, 0.f), 1.f); }
-
 // Bit c set when channel c lies in [0,1]: torch.clamp's gradient passes at both bounds.
 __device__ __forceinline__ uint32_t clip_bits(const float4& v) {
   return (uint32_t)(v.x >= 0.f && v.x <= 1.f) | ((uint32_t)(v.y >= 0.f && v.y <= 1.f) << 1) |
          ((uint32_t)(v.z >= 0.f && v.z <= 1.f) << 2) | ((uint32_t)(v.w >= 0.f && v.w <= 1.f) << 3);
-}
-
-__device__ __forceinline__ float4 clip4(const float4& v) {
-  return make_float4(clip01(v.x), clip01(v.y), clip01(v.z), clip01(v.w));
 }
 
 __device__ __forceinline__ bool occupied(const uint32_t* occ, uint32_t cell) {
@@ -349,7 +343,10 @@ __device__ __forceinline__ bool classify(const KParams& P, const Ray& R, int k) 
   return inb && occupied(P.occ, cell);
 }
 
-__device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.f, 1.f); }
+// torch.clamp(v, 0, 1): NaN stays NaN (v_med3 alone would return a bound)
+__device__ __forceinline__ float clamp01(float v) {
+  return __builtin_isnan(v) ? v : __builtin_amdgcn_fmed3f(v, 0.f, 1.f);
+}
 
 // Clipped value and clip-gradient bit of one channel (bit set iff 0 <= raw <= 1).
 __device__ __forceinline__ float clip_ch(float raw, uint32_t& bits, int bit) {

@@ -1,0 +1,834 @@
+// pxl_ops.hip -- the per-function kernels behind the reference's individual call signatures
+// (src/ray_sampling.py, src/grid_functions.py, scripts/train.py) and the optimiser step.
+//
+// These materialise the same intermediate tensors the reference functions return, so that a
+// caller composing them (as scripts/train.py does) gets the reference's results; the fused
+// path (pxl_render.hip) is what a training loop should call.  Elementwise kernels use
+// grid-stride loops; anything that accumulates into a grid uses float atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "pxl.h"
+#include "pxl_device.h"
+#include "pxl_host.h"
+
+using pxl_host::fail;
+using pxl_host::launched;
+using pxl_host::stride_blocks;
+
+namespace pxl {
+namespace {
+
+#define PXL_GRID_STRIDE(i, n)                                                       \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)(n); \
+       i += (int64_t)gridDim.x * blockDim.x)
+
+// Python's % on int64 (result has the sign of the divisor): src/grid_functions.py:75-77.
+__device__ __forceinline__ int64_t pymod(int64_t a, int64_t m) {
+  const int64_t r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+// float -> int64 as torch's .to(torch.long) for the finite values the kernels see;
+// non-finite or huge inputs map to a value that fails every bounds test.
+__device__ __forceinline__ int64_t to_i64(float x) {
+  return (fabsf(x) < 9.0e18f) ? (int64_t)x : INT64_MIN / 2;
+}
+
+struct Dims {
+  int64_t X, Y, Z;
+};
+
+__device__ __forceinline__ int64_t cell_of(const Dims& d, int64_t i, int64_t j, int64_t k) {
+  return (i * d.Y + j) * d.Z + k;
+}
+
+__device__ __forceinline__ void atomic_add4(float* p, const float4& g) {
+  if (g.x != 0.f) unsafeAtomicAdd(p + 0, g.x);
+  if (g.y != 0.f) unsafeAtomicAdd(p + 1, g.y);
+  if (g.z != 0.f) unsafeAtomicAdd(p + 2, g.z);
+  if (g.w != 0.f) unsafeAtomicAdd(p + 3, g.w);
+}
+
+// ---------------------------------------------------------------- samples, normalisation
+
+__global__ void k_sample_points(const float* __restrict__ o, int64_t rays_per_origin,
+                                const float* __restrict__ d, int64_t n_rays, int S, float delta,
+                                float near, float* __restrict__ out) {
+  PXL_GRID_STRIDE(i, n_rays * S) {
+    const int64_t r = i / S;
+    const int k = (int)(i - r * S) + 1;
+    const int64_t oi = r / rays_per_origin;
+    float t = delta * (float)k;
+    t = t + near;
+    out[3 * i + 0] = o[3 * oi + 0] + d[3 * r + 0] * t;
+    out[3 * i + 1] = o[3 * oi + 1] + d[3 * r + 1] * t;
+    out[3 * i + 2] = o[3 * oi + 2] + d[3 * r + 2] * t;
+  }
+}
+
+__global__ void k_init_min(float* out3) {
+  if (threadIdx.x < 3) out3[threadIdx.x] = INFINITY;
+}
+
+__device__ __forceinline__ void atomic_min_f32(float* a, float v) {
+  unsigned int* ai = reinterpret_cast<unsigned int*>(a);
+  unsigned int old = *ai;
+  while (v < __uint_as_float(old)) {
+    const unsigned int prev = atomicCAS(ai, old, __float_as_uint(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+__global__ void k_coord_min(const float* __restrict__ c, int64_t n, float* out3) {
+  float m0 = INFINITY, m1 = INFINITY, m2 = INFINITY;
+  PXL_GRID_STRIDE(i, n) {
+    m0 = fminf(m0, c[3 * i]);
+    m1 = fminf(m1, c[3 * i + 1]);
+    m2 = fminf(m2, c[3 * i + 2]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = fminf(m0, __shfl_xor(m0, off));
+    m1 = fminf(m1, __shfl_xor(m1, off));
+    m2 = fminf(m2, __shfl_xor(m2, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_min_f32(out3 + 0, m0);
+    atomic_min_f32(out3 + 1, m1);
+    atomic_min_f32(out3 + 2, m2);
+  }
+}
+
+__global__ void k_normalize(const float* __restrict__ p, int64_t n, const float* __restrict__ m3,
+                            float pd, float inv, int exact, float* __restrict__ out) {
+  const float g0 = m3[0], g1 = m3[1], g2 = m3[2];
+  PXL_GRID_STRIDE(i, n) {
+    const float d0 = p[3 * i] - g0, d1 = p[3 * i + 1] - g1, d2 = p[3 * i + 2] - g2;
+    out[3 * i + 0] = exact ? __fdiv_rn(d0, pd) : d0 * inv;
+    out[3 * i + 1] = exact ? __fdiv_rn(d1, pd) : d1 * inv;
+    out[3 * i + 2] = exact ? __fdiv_rn(d2, pd) : d2 * inv;
+  }
+}
+
+// ---------------------------------------------------------------- bounds, wrap, nearest
+
+__global__ void k_in_bounds_i64(const int64_t* __restrict__ idx, int64_t n, Dims d,
+                                uint8_t* __restrict__ mask) {
+  PXL_GRID_STRIDE(i, n) {
+    const int64_t a = idx[3 * i], b = idx[3 * i + 1], c = idx[3 * i + 2];
+    mask[i] = (a < d.X) & (a >= 0) & (b < d.Y) & (b >= 0) & (c < d.Z) & (c >= 0);
+  }
+}
+
+__global__ void k_in_bounds_f32(const float* __restrict__ idx, int64_t n, Dims d,
+                                uint8_t* __restrict__ mask) {
+  const float fx = (float)d.X, fy = (float)d.Y, fz = (float)d.Z;
+  PXL_GRID_STRIDE(i, n) {
+    const float a = idx[3 * i], b = idx[3 * i + 1], c = idx[3 * i + 2];
+    mask[i] = (a < fx) & (a >= 0.f) & (b < fy) & (b >= 0.f) & (c < fz) & (c >= 0.f);
+  }
+}
+
+__global__ void k_wrap(int64_t* __restrict__ idx, int64_t n, Dims d) {
+  PXL_GRID_STRIDE(i, n) {
+    idx[3 * i] = pymod(idx[3 * i], d.X);
+    idx[3 * i + 1] = pymod(idx[3 * i + 1], d.Y);
+    idx[3 * i + 2] = pymod(idx[3 * i + 2], d.Z);
+  }
+}
+
+// idx = round-half-even(n).long(); mask = in-bounds; cell = wrapped idx
+__device__ __forceinline__ int64_t nearest_wrapped(const float* n, int64_t i, const Dims& d,
+                                                   bool& inb) {
+  const int64_t a = to_i64(__builtin_rintf(n[3 * i])), b = to_i64(__builtin_rintf(n[3 * i + 1])),
+                c = to_i64(__builtin_rintf(n[3 * i + 2]));
+  inb = a >= 0 && a < d.X && b >= 0 && b < d.Y && c >= 0 && c < d.Z;
+  return cell_of(d, pymod(a, d.X), pymod(b, d.Y), pymod(c, d.Z));
+}
+
+__global__ void k_nearest_gather(const float* __restrict__ n, int64_t N, Dims d,
+                                 const float4* __restrict__ grid, float4* __restrict__ vals,
+                                 uint8_t* __restrict__ mask) {
+  PXL_GRID_STRIDE(i, N) {
+    bool inb;
+    const int64_t cell = nearest_wrapped(n, i, d, inb);
+    vals[i] = grid[cell];
+    if (mask) mask[i] = inb;
+  }
+}
+
+__global__ void k_nearest_scatter(const float* __restrict__ n, int64_t N, Dims d,
+                                  const float4* __restrict__ g, float* __restrict__ grad) {
+  PXL_GRID_STRIDE(i, N) {
+    bool inb;
+    const int64_t cell = nearest_wrapped(n, i, d, inb);
+    atomic_add4(grad + 4 * cell, g[i]);
+  }
+}
+
+// ---------------------------------------------------------------- corners, trilinear
+
+// ceil/floor of each coordinate as int64 (src/grid_functions.py:230-244), [a][0] = ceil.
+__device__ __forceinline__ void ceil_floor(const float* n, int64_t i, int64_t cf[3][2]) {
+  for (int a = 0; a < 3; ++a) {
+    cf[a][0] = to_i64(ceilf(n[3 * i + a]));
+    cf[a][1] = to_i64(floorf(n[3 * i + a]));
+  }
+}
+
+__global__ void k_corner_indices(const float* __restrict__ n, int64_t N, int64_t* __restrict__ out) {
+  PXL_GRID_STRIDE(i, N) {
+    int64_t cf[3][2];
+    ceil_floor(n, i, cf);
+    int64_t* o = out + 24 * i;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      o[3 * j + 0] = cf[0][(j >> 2) & 1];
+      o[3 * j + 1] = cf[1][(j >> 1) & 1];
+      o[3 * j + 2] = cf[2][j & 1];
+    }
+  }
+}
+
+__device__ __forceinline__ bool float_in_bounds(const float* n, int64_t i, const Dims& d) {
+  const float a = n[3 * i], b = n[3 * i + 1], c = n[3 * i + 2];
+  return a < (float)d.X && a >= 0.f && b < (float)d.Y && b >= 0.f && c < (float)d.Z && c >= 0.f;
+}
+
+__global__ void k_collect(const float* __restrict__ n, int64_t N, Dims d,
+                          const float4* __restrict__ grid, float4* __restrict__ out,
+                          uint8_t* __restrict__ mask) {
+  PXL_GRID_STRIDE(i, N) {
+    int64_t cf[3][2];
+    ceil_floor(n, i, cf);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      out[8 * i + j] = grid[cell_of(d, pymod(cf[0][(j >> 2) & 1], d.X), pymod(cf[1][(j >> 1) & 1], d.Y),
+                                    pymod(cf[2][j & 1], d.Z))];
+    if (mask) mask[i] = float_in_bounds(n, i, d);
+  }
+}
+
+__global__ void k_collect_bwd(const float* __restrict__ n, int64_t N, Dims d,
+                              const float4* __restrict__ g, float* __restrict__ grad) {
+  PXL_GRID_STRIDE(i, N) {
+    int64_t cf[3][2];
+    ceil_floor(n, i, cf);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      atomic_add4(grad + 4 * cell_of(d, pymod(cf[0][(j >> 2) & 1], d.X),
+                                     pymod(cf[1][(j >> 1) & 1], d.Y), pymod(cf[2][j & 1], d.Z)),
+                  g[8 * i + j]);
+  }
+}
+
+__device__ __forceinline__ int64_t corner_cell(const int64_t* c, int64_t i, int j, const Dims& d) {
+  const int64_t* p = c + 24 * i + 3 * j;
+  return cell_of(d, pymod(p[0], d.X), pymod(p[1], d.Y), pymod(p[2], d.Z));
+}
+
+__device__ __forceinline__ float4 lerp_ref(const float4& a, const float4& b, float f, float of) {
+  return make_float4((a.x * f) + (b.x * of), (a.y * f) + (b.y * of), (a.z * f) + (b.z * of),
+                     (a.w * f) + (b.w * of));
+}
+
+// src/grid_functions.py:29-42 in its own op order: x between corners 0-3 (weight frac) and
+// 4-7 (1-frac), then y, then z.
+__global__ void k_trilinear(const float* __restrict__ n, const int64_t* __restrict__ c, int64_t N,
+                            Dims d, const float4* __restrict__ grid, float4* __restrict__ out) {
+  PXL_GRID_STRIDE(i, N) {
+    const float fx = n[3 * i] - truncf(n[3 * i]);
+    const float fy = n[3 * i + 1] - truncf(n[3 * i + 1]);
+    const float fz = n[3 * i + 2] - truncf(n[3 * i + 2]);
+    float4 q[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      q[a] = lerp_ref(grid[corner_cell(c, i, a, d)], grid[corner_cell(c, i, 4 + a, d)], fx, 1.f - fx);
+    const float4 m0 = lerp_ref(q[0], q[2], fy, 1.f - fy);
+    const float4 m1 = lerp_ref(q[1], q[3], fy, 1.f - fy);
+    out[i] = lerp_ref(m0, m1, fz, 1.f - fz);
+  }
+}
+
+__global__ void k_trilinear_bwd(const float* __restrict__ n, const int64_t* __restrict__ c,
+                                int64_t N, Dims d, const float4* __restrict__ g,
+                                float* __restrict__ grad) {
+  PXL_GRID_STRIDE(i, N) {
+    const float fx = n[3 * i] - truncf(n[3 * i]);
+    const float fy = n[3 * i + 1] - truncf(n[3 * i + 1]);
+    const float fz = n[3 * i + 2] - truncf(n[3 * i + 2]);
+    const float4 gi = g[i];
+    const float wz[2] = {fz, 1.f - fz}, wy[2] = {fy, 1.f - fy}, wx[2] = {fx, 1.f - fx};
+    // corner j = 4*bx + 2*by + bz: d out / d corner = wz[bz] * wy[by] * wx[bx] (autograd order)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float sz = wz[j & 1], sy = wy[(j >> 1) & 1], sx = wx[(j >> 2) & 1];
+      const float4 gz = make_float4(gi.x * sz, gi.y * sz, gi.z * sz, gi.w * sz);
+      const float4 gy = make_float4(gz.x * sy, gz.y * sy, gz.z * sy, gz.w * sy);
+      atomic_add4(grad + 4 * corner_cell(c, i, j, d),
+                  make_float4(gy.x * sx, gy.y * sx, gy.z * sx, gy.w * sx));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- compositing
+
+// one wave per ray; samples (R,S,4)
+__global__ __launch_bounds__(kBlock) void k_composite_fwd(const float4* __restrict__ smp, int64_t R,
+                                                           int S, float4* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (r >= R) return;
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  for (int c0 = 0; c0 < S; c0 += kWave) {
+    const int k = c0 + lane;
+    const float4 v = k < S ? smp[r * S + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float incl = wave_prefix_mul(1.f - v.w);
+    const float w = v.w * (T * wave_shift_up(incl, 1.f));
+    cr += v.x * w;
+    cg += v.y * w;
+    cb += v.z * w;
+    ca += w;
+    T = T * readlane(incl, 63);
+  }
+  cr = wave_sum(cr);
+  cg = wave_sum(cg);
+  cb = wave_sum(cb);
+  ca = wave_sum(ca);
+  if (lane == 0) out[r] = make_float4(cr, cg, cb, ca);
+}
+
+constexpr int kMaxCompositeChunks = 1024;  // S <= 65536
+
+__global__ __launch_bounds__(kBlock) void k_composite_bwd(const float4* __restrict__ smp, int64_t R,
+                                                           int S, const float4* __restrict__ gout,
+                                                           float4* __restrict__ gsmp) {
+  __shared__ float s_T[kWavesPerBlock][kMaxCompositeChunks];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  if (r >= R) return;
+  const int nch = (S + 63) >> 6;
+  float T = 1.f;
+  for (int c = 0; c < nch; ++c) {  // transmittance at every chunk start
+    const int k = (c << 6) + lane;
+    const float a = k < S ? smp[r * S + k].w : 0.f;
+    if (lane == 0) s_T[wv][c] = T;
+    T = T * readlane(wave_prefix_mul(1.f - a), 63);
+  }
+  wave_lds_sync();
+  const float4 g = gout[r];
+  float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c = nch - 1; c >= 0; --c) {
+    const int k = (c << 6) + lane;
+    const bool valid = k < S;
+    const float4 v = valid ? smp[r * S + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float incl = wave_prefix_mul(1.f - v.w);
+    const float Ti = s_T[wv][c] * wave_shift_up(incl, 1.f);
+    Affine4 f{v.w * v.x, v.w * v.y, v.w * v.z, v.w, 1.f - v.w};
+    if (lane == 63) {
+      f.a0 = f.a0 + f.b * Qn.x;
+      f.a1 = f.a1 + f.b * Qn.y;
+      f.a2 = f.a2 + f.b * Qn.z;
+      f.a3 = f.a3 + f.b * Qn.w;
+      f.b = 0.f;
+    }
+    wave_suffix_affine(f, lane);
+    const float s0 = wave_shift_down(f.a0, Qn.x), s1 = wave_shift_down(f.a1, Qn.y);
+    const float s2 = wave_shift_down(f.a2, Qn.z), s3 = wave_shift_down(f.a3, Qn.w);
+    if (valid) {
+      const float w = v.w * Ti;
+      gsmp[r * S + k] = make_float4(
+          w * g.x, w * g.y, w * g.z,
+          Ti * (g.x * (v.x - s0) + g.y * (v.y - s1) + g.z * (v.z - s2) + g.w * (1.f - s3)));
+    }
+    Qn = make_float4(readlane(f.a0, 0), readlane(f.a1, 0), readlane(f.a2, 0), readlane(f.a3, 0));
+  }
+}
+
+// ---------------------------------------------------------------- grid layout, pooling
+
+__global__ void k_grid_coords(Dims d, float cx, float cy, float cz, float pd,
+                              float* __restrict__ coords, int64_t* __restrict__ mesh) {
+  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
+    const int64_t k = L % d.Z, j = (L / d.Z) % d.Y, i = L / (d.Y * d.Z);
+    coords[3 * L + 0] = (((float)i - cx) + 1.f) * pd;
+    coords[3 * L + 1] = (((float)j - cy) + 1.f) * pd;
+    coords[3 * L + 2] = (((float)k - cz) + 1.f) * pd;
+    if (mesh) {
+      mesh[3 * L + 0] = i;
+      mesh[3 * L + 1] = j;
+      mesh[3 * L + 2] = k;
+    }
+  }
+}
+
+// one block per output cell: the k^3 window summed by all threads, then reduced
+__global__ __launch_bounds__(256) void k_avg_pool(const float4* __restrict__ in, Dims d, Dims o,
+                                                   int k, int s, float4* __restrict__ out) {
+  __shared__ float4 red[256 / kWave];
+  const int64_t L = blockIdx.x;
+  const int64_t zo = L % o.Z, yo = (L / o.Z) % o.Y, xo = L / (o.Y * o.Z);
+  const int64_t kk = (int64_t)k * k * k;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t w = threadIdx.x; w < kk; w += blockDim.x) {
+    const int64_t dz = w % k, dy = (w / k) % k, dx = w / ((int64_t)k * k);
+    const float4 v = in[cell_of(d, xo * s + dx, yo * s + dy, zo * s + dz)];
+    acc.x += v.x;
+    acc.y += v.y;
+    acc.z += v.z;
+    acc.w += v.w;
+  }
+  acc.x = wave_sum(acc.x);
+  acc.y = wave_sum(acc.y);
+  acc.z = wave_sum(acc.z);
+  acc.w = wave_sum(acc.w);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float4 t = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
+      t.x += red[i].x;
+      t.y += red[i].y;
+      t.z += red[i].z;
+      t.w += red[i].w;
+    }
+    const float q = (float)kk;
+    out[L] = make_float4(t.x / q, t.y / q, t.z / q, t.w / q);
+  }
+}
+
+// thread per output cell, for small windows
+__global__ void k_avg_pool_small(const float4* __restrict__ in, Dims d, Dims o, int k, int s,
+                                 float4* __restrict__ out) {
+  const float q = (float)((int64_t)k * k * k);
+  PXL_GRID_STRIDE(L, o.X * o.Y * o.Z) {
+    const int64_t zo = L % o.Z, yo = (L / o.Z) % o.Y, xo = L / (o.Y * o.Z);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int dx = 0; dx < k; ++dx)
+      for (int dy = 0; dy < k; ++dy)
+        for (int dz = 0; dz < k; ++dz) {
+          const float4 v = in[cell_of(d, xo * s + dx, yo * s + dy, zo * s + dz)];
+          acc.x += v.x;
+          acc.y += v.y;
+          acc.z += v.z;
+          acc.w += v.w;
+        }
+    out[L] = make_float4(acc.x / q, acc.y / q, acc.z / q, acc.w / q);
+  }
+}
+
+// windows o with o*s <= x < o*s + k, clipped to [0, n_out)
+__device__ __forceinline__ void window_range(int64_t x, int k, int s, int64_t n_out, int64_t& lo,
+                                             int64_t& hi) {
+  lo = x - k + 1 <= 0 ? 0 : (x - k + 1 + s - 1) / s;
+  hi = x / s;
+  if (hi > n_out - 1) hi = n_out - 1;
+}
+
+__global__ void k_avg_pool_bwd(const float4* __restrict__ gout, Dims d, Dims o, int k, int s,
+                               float4* __restrict__ grad) {
+  const float q = (float)((int64_t)k * k * k);
+  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
+    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
+    int64_t x0, x1, y0, y1, z0, z1;
+    window_range(x, k, s, o.X, x0, x1);
+    window_range(y, k, s, o.Y, y0, y1);
+    window_range(z, k, s, o.Z, z0, z1);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t a = x0; a <= x1; ++a)
+      for (int64_t b = y0; b <= y1; ++b)
+        for (int64_t c = z0; c <= z1; ++c) {
+          const float4 g = gout[cell_of(o, a, b, c)];
+          acc.x += g.x / q;
+          acc.y += g.y / q;
+          acc.z += g.z / q;
+          acc.w += g.w / q;
+        }
+    float4 cur = grad[L];
+    cur.x += acc.x;
+    cur.y += acc.y;
+    cur.z += acc.z;
+    cur.w += acc.w;
+    grad[L] = cur;
+  }
+}
+
+// ---------------------------------------------------------------- total variation
+
+__device__ __forceinline__ double sq4(const float4& a, const float4& b) {
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z, dw = a.w - b.w;
+  return (double)(dx * dx) + (double)(dy * dy) + (double)(dz * dz) + (double)(dw * dw);
+}
+
+__global__ __launch_bounds__(256) void k_tv_sumsq(const float4* __restrict__ g, Dims d,
+                                                  double* __restrict__ sum) {
+  double acc = 0.0;
+  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
+    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
+    const float4 v = g[L];
+    if (y + 1 < d.Y) acc += sq4(v, g[L + d.Z]);
+    if (z + 1 < d.Z) acc += sq4(v, g[L + 1]);
+    if (x + 1 < d.X) acc += sq4(v, g[L + d.Y * d.Z]);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(sum, acc);
+}
+
+__global__ void k_tv_bwd(const float4* __restrict__ g, Dims d, const double* __restrict__ sum,
+                         float weight, float* __restrict__ grad) {
+  // d/dx of weight*sqrt(S): weight / sqrt(S) * sum over neighbours of (x - neighbour)
+  const float scale = weight / (float)sqrt(*sum);
+  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
+    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
+    const float4 v = g[L];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t nb[6] = {y + 1 < d.Y ? L + d.Z : -1, y > 0 ? L - d.Z : -1,
+                           z + 1 < d.Z ? L + 1 : -1,   z > 0 ? L - 1 : -1,
+                           x + 1 < d.X ? L + d.Y * d.Z : -1, x > 0 ? L - d.Y * d.Z : -1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if (nb[q] < 0) continue;
+      const float4 u = g[nb[q]];
+      acc.x += v.x - u.x;
+      acc.y += v.y - u.y;
+      acc.z += v.z - u.z;
+      acc.w += v.w - u.w;
+    }
+    float* p = grad + 4 * L;
+    p[0] += scale * acc.x;
+    p[1] += scale * acc.y;
+    p[2] += scale * acc.z;
+    p[3] += scale * acc.w;
+  }
+}
+
+// ---------------------------------------------------------------- Adam
+
+__device__ __forceinline__ uint16_t to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float* __restrict__ gabs, int64_t n, float w1,
+                       float beta2, float w2, float bc2_sqrt, float eps, float neg_step,
+                       void* __restrict__ half_out, int half_dtype) {
+  PXL_GRID_STRIDE(i, n) {
+    const float gi = g[i];
+    // exp_avg.lerp_(grad, 1 - beta1): weight < 0.5 form  m + w * (g - m)
+    const float mi = m[i] + w1 * (gi - m[i]);
+    const float vi = (v[i] * beta2) + (w2 * gi) * gi;  // addcmul: value * t1 * t2
+    const float denom = (__builtin_sqrtf(vi) / bc2_sqrt) + eps;
+    const float pi = p[i] + neg_step * (mi / denom);
+    m[i] = mi;
+    v[i] = vi;
+    p[i] = pi;
+    if (gabs) gabs[i] += fabsf(gi);
+    if (half_out) {
+      if (half_dtype == PXL_DTYPE_BF16)
+        reinterpret_cast<uint16_t*>(half_out)[i] = to_bf16_rne(pi);
+      else
+        reinterpret_cast<_Float16*>(half_out)[i] = (_Float16)pi;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace pxl
+
+// ============================================================================= host ABI
+
+namespace {
+
+using pxl::Dims;
+
+int need(bool ok, const char* what) { return ok ? PXL_OK : fail(PXL_E_INVALID, "%s", what); }
+
+int f32_grid(const pxl_grid* g, Dims& d) {
+  int rc = pxl_host::check_grid(g);
+  if (rc) return rc;
+  if (g->dtype != PXL_DTYPE_F32) return fail(PXL_E_DTYPE, "this entry point takes a float32 grid");
+  d = Dims{g->dims[0], g->dims[1], g->dims[2]};
+  return PXL_OK;
+}
+
+int dims3(const int64_t dims[3], Dims& d) {
+  if (!dims) return fail(PXL_E_INVALID, "dims is NULL");
+  int rc = pxl_host::check_dims(dims[0], dims[1], dims[2]);
+  if (rc) return rc;
+  d = Dims{dims[0], dims[1], dims[2]};
+  return PXL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pxl_sample_points(const float* origins, int64_t rays_per_origin, const float* dirs,
+                      int64_t n_rays, int32_t n_samples, float delta, float near, float* out,
+                      void* stream) {
+  int rc;
+  if ((rc = need(origins && dirs && out, "origins, dirs and out are required")) ||
+      (rc = need(n_rays >= 0 && n_samples >= 0 && rays_per_origin > 0, "negative size")))
+    return rc;
+  if (n_rays == 0 || n_samples == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_sample_points, dim3(stride_blocks(n_rays * n_samples, 256)), dim3(256), 0,
+                     (hipStream_t)stream, origins, rays_per_origin, dirs, n_rays, n_samples, delta,
+                     near, out);
+  return launched("pxl_sample_points");
+}
+
+int pxl_coord_min(const float* coords, int64_t n, float* out3, void* stream) {
+  int rc;
+  if ((rc = need(coords && out3 && n > 0, "coords, out3 and n > 0 are required"))) return rc;
+  hipLaunchKernelGGL(pxl::k_init_min, dim3(1), dim3(64), 0, (hipStream_t)stream, out3);
+  hipLaunchKernelGGL(pxl::k_coord_min, dim3(stride_blocks(n, 256, 2048)), dim3(256), 0,
+                     (hipStream_t)stream, coords, n, out3);
+  return launched("pxl_coord_min");
+}
+
+int pxl_normalize_points(const float* p, int64_t n, const float* min3, float pd, float inv_pd,
+                         int32_t flags, float* out, void* stream) {
+  int rc;
+  if ((rc = need(p && min3 && out && n >= 0, "p, min3, out required and n >= 0")) ||
+      (rc = need((flags & ~PXL_FLAG_EXACT_DIV) == 0, "unknown flags")))
+    return rc;
+  if (n == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_normalize, dim3(stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     p, n, min3, pd, inv_pd, (flags & PXL_FLAG_EXACT_DIV) ? 1 : 0, out);
+  return launched("pxl_normalize_points");
+}
+
+int pxl_in_bounds(const void* idx, int32_t is_float, int64_t n, const int64_t dims[3],
+                  uint8_t* mask, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = need(idx && mask && n >= 0, "idx, mask required and n >= 0")) || (rc = dims3(dims, d)))
+    return rc;
+  if (n == 0) return PXL_OK;
+  if (is_float)
+    hipLaunchKernelGGL(pxl::k_in_bounds_f32, dim3(stride_blocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float*)idx, n, d, mask);
+  else
+    hipLaunchKernelGGL(pxl::k_in_bounds_i64, dim3(stride_blocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const int64_t*)idx, n, d, mask);
+  return launched("pxl_in_bounds");
+}
+
+int pxl_wrap_indices(int64_t* idx, int64_t n, const int64_t dims[3], void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = need(idx && n >= 0, "idx required and n >= 0")) || (rc = dims3(dims, d))) return rc;
+  if (n == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_wrap, dim3(stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, idx,
+                     n, d);
+  return launched("pxl_wrap_indices");
+}
+
+int pxl_nearest_gather(const float* n, int64_t N, const pxl_grid* grid, float* vals,
+                       uint8_t* mask, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) || (rc = need(n && vals && N >= 0, "n, vals required and N >= 0")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_nearest_gather, dim3(stride_blocks(N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, N, d, (const float4*)grid->data, (float4*)vals, mask);
+  return launched("pxl_nearest_gather");
+}
+
+int pxl_nearest_scatter(const float* n, int64_t N, const pxl_grid* grid, const float* grad_vals,
+                        float* grad_grid, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) ||
+      (rc = need(n && grad_vals && grad_grid && N >= 0, "n, grad_vals, grad_grid required")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_nearest_scatter, dim3(stride_blocks(N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, N, d, (const float4*)grad_vals, grad_grid);
+  return launched("pxl_nearest_scatter");
+}
+
+int pxl_corner_indices(const float* n, int64_t N, int64_t* corners, void* stream) {
+  int rc;
+  if ((rc = need(n && corners && N >= 0, "n, corners required and N >= 0"))) return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_corner_indices, dim3(stride_blocks(N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, N, corners);
+  return launched("pxl_corner_indices");
+}
+
+int pxl_collect_cells(const float* n, int64_t N, const pxl_grid* grid, float* out, uint8_t* mask,
+                      void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) || (rc = need(n && out && N >= 0, "n, out required and N >= 0")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_collect, dim3(stride_blocks(N, 256)), dim3(256), 0, (hipStream_t)stream,
+                     n, N, d, (const float4*)grid->data, (float4*)out, mask);
+  return launched("pxl_collect_cells");
+}
+
+int pxl_collect_cells_bwd(const float* n, int64_t N, const pxl_grid* grid, const float* grad_out,
+                          float* grad_grid, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) ||
+      (rc = need(n && grad_out && grad_grid && N >= 0, "n, grad_out, grad_grid required")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_collect_bwd, dim3(stride_blocks(N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, N, d, (const float4*)grad_out, grad_grid);
+  return launched("pxl_collect_cells_bwd");
+}
+
+int pxl_trilinear(const float* n, const int64_t* corners, int64_t N, const pxl_grid* grid,
+                  float* out, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) ||
+      (rc = need(n && corners && out && N >= 0, "n, corners, out required and N >= 0")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_trilinear, dim3(stride_blocks(N, 256)), dim3(256), 0, (hipStream_t)stream,
+                     n, corners, N, d, (const float4*)grid->data, (float4*)out);
+  return launched("pxl_trilinear");
+}
+
+int pxl_trilinear_bwd(const float* n, const int64_t* corners, int64_t N, const pxl_grid* grid,
+                      const float* grad_out, float* grad_grid, void* stream) {
+  int rc;
+  Dims d;
+  if ((rc = f32_grid(grid, d)) ||
+      (rc = need(n && corners && grad_out && grad_grid && N >= 0, "pointers required")))
+    return rc;
+  if (N == 0) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_trilinear_bwd, dim3(stride_blocks(N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, corners, N, d, (const float4*)grad_out, grad_grid);
+  return launched("pxl_trilinear_bwd");
+}
+
+int pxl_composite_fwd(const float* samples, int64_t R, int32_t S, float* out, void* stream) {
+  int rc;
+  if ((rc = need(samples && out && R >= 0 && S >= 0, "samples, out required; sizes >= 0")))
+    return rc;
+  if (R == 0) return PXL_OK;
+  if (S > 64 * pxl::kMaxCompositeChunks) return fail(PXL_E_SIZE, "S > 65536");
+  for (int64_t off = 0; off < R; off += int64_t(1) << 26) {
+    const int64_t n = std::min<int64_t>(R - off, int64_t(1) << 26);
+    hipLaunchKernelGGL(pxl::k_composite_fwd, dim3((unsigned)((n + 3) / 4)), dim3(pxl::kBlock), 0,
+                       (hipStream_t)stream, (const float4*)samples + off * S, n, S,
+                       (float4*)out + off);
+  }
+  return launched("pxl_composite_fwd");
+}
+
+int pxl_composite_bwd(const float* samples, int64_t R, int32_t S, const float* grad_out,
+                      float* grad_samples, void* stream) {
+  int rc;
+  if ((rc = need(samples && grad_out && grad_samples && R >= 0 && S >= 0, "pointers required")))
+    return rc;
+  if (R == 0 || S == 0) return PXL_OK;
+  if (S > 64 * pxl::kMaxCompositeChunks) return fail(PXL_E_SIZE, "S > 65536");
+  for (int64_t off = 0; off < R; off += int64_t(1) << 26) {
+    const int64_t n = std::min<int64_t>(R - off, int64_t(1) << 26);
+    hipLaunchKernelGGL(pxl::k_composite_bwd, dim3((unsigned)((n + 3) / 4)), dim3(pxl::kBlock), 0,
+                       (hipStream_t)stream, (const float4*)samples + off * S, n, S,
+                       (const float4*)grad_out + off, (float4*)grad_samples + off * S);
+  }
+  return launched("pxl_composite_bwd");
+}
+
+int pxl_grid_coords(int64_t X, int64_t Y, int64_t Z, float pd, float* coords, int64_t* mesh,
+                    void* stream) {
+  int rc;
+  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(coords != nullptr, "coords is NULL")))
+    return rc;
+  auto c = [](int64_t s) { return (float)((s + 1) / 2); };  // ceil(s/2)
+  hipLaunchKernelGGL(pxl::k_grid_coords, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
+                     (hipStream_t)stream, Dims{X, Y, Z}, c(X), c(Y), c(Z), pd, coords, mesh);
+  return launched("pxl_grid_coords");
+}
+
+int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s,
+                   float* out, void* stream) {
+  int rc;
+  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grid && out, "grid and out required")) ||
+      (rc = need(k >= 1 && s >= 1 && k <= X && k <= Y && k <= Z, "need 1 <= k <= dims, s >= 1")))
+    return rc;
+  const Dims o{(X - k) / s + 1, (Y - k) / s + 1, (Z - k) / s + 1};
+  const int64_t n_out = o.X * o.Y * o.Z;
+  if ((int64_t)k * k * k <= 512 || n_out > (int64_t(1) << 20))
+    hipLaunchKernelGGL(pxl::k_avg_pool_small, dim3(stride_blocks(n_out, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, o, k, s, (float4*)out);
+  else
+    hipLaunchKernelGGL(pxl::k_avg_pool, dim3((unsigned)n_out), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)grid, Dims{X, Y, Z}, o, k, s, (float4*)out);
+  return launched("pxl_avg_pool3d");
+}
+
+int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
+                       int32_t s, float* grad_grid, void* stream) {
+  int rc;
+  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grad_out && grad_grid, "pointers required")) ||
+      (rc = need(k >= 1 && s >= 1 && k <= X && k <= Y && k <= Z, "need 1 <= k <= dims, s >= 1")))
+    return rc;
+  const Dims o{(X - k) / s + 1, (Y - k) / s + 1, (Z - k) / s + 1};
+  hipLaunchKernelGGL(pxl::k_avg_pool_bwd, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float4*)grad_out, Dims{X, Y, Z}, o, k, s,
+                     (float4*)grad_grid);
+  return launched("pxl_avg_pool3d_bwd");
+}
+
+int pxl_tv_sumsq(const float* grid, int64_t X, int64_t Y, int64_t Z, double* sum_sq, void* stream) {
+  int rc;
+  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grid && sum_sq, "grid and sum_sq required")))
+    return rc;
+  hipLaunchKernelGGL(pxl::k_tv_sumsq, dim3(stride_blocks(X * Y * Z, 256, 4096)), dim3(256), 0,
+                     (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq);
+  return launched("pxl_tv_sumsq");
+}
+
+int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double* sum_sq,
+               float weight, float* grad_grid, void* stream) {
+  int rc;
+  if ((rc = pxl_host::check_dims(X, Y, Z)) ||
+      (rc = need(grid && sum_sq && grad_grid, "grid, sum_sq and grad_grid required")))
+    return rc;
+  hipLaunchKernelGGL(pxl::k_tv_bwd, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq, weight,
+                     grad_grid);
+  return launched("pxl_tv_bwd");
+}
+
+int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                  float* grad_abs, int64_t n, double lr, double beta1, double beta2, double eps,
+                  int64_t step, void* half_out, int32_t half_dtype, void* stream) {
+  int rc;
+  if ((rc = need(param && grad && exp_avg && exp_avg_sq && n >= 0, "param, grad, moments required")) ||
+      (rc = need(step >= 1, "step counts from 1")) ||
+      (rc = need(!half_out || half_dtype == PXL_DTYPE_BF16 || half_dtype == PXL_DTYPE_F16,
+                 "half_dtype must be PXL_DTYPE_BF16 or PXL_DTYPE_F16")))
+    return rc;
+  if (n == 0) return PXL_OK;
+  // scalar terms in double, as torch.optim.Adam forms them in Python
+  const double bc1 = 1.0 - std::pow(beta1, (double)step);
+  const double bc2 = 1.0 - std::pow(beta2, (double)step);
+  const float neg_step = (float)(-(lr / bc1));
+  const float bc2_sqrt = (float)std::pow(bc2, 0.5);
+  hipLaunchKernelGGL(pxl::k_adam, dim3(stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     param, grad, exp_avg, exp_avg_sq, grad_abs, n, (float)(1.0 - beta1),
+                     (float)beta2, (float)(1.0 - beta2), bc2_sqrt, (float)eps, neg_step, half_out,
+                     half_dtype);
+  return launched("pxl_adam_step");
+}
+
+}  // extern "C"

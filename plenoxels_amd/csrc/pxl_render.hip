@@ -23,6 +23,7 @@
 
 #include "pxl.h"
 #include "pxl_device.h"
+#include "pxl_host.h"
 
 namespace pxl {
 
@@ -119,14 +120,25 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
   }
 }
 
+// Beta regulariser of scripts/train.py:170-174: f(a) = log(a + eps) - log(1 - a + eps) per
+// sample, eps = 1e-4, and its derivative.
+constexpr float kBetaEps = 1e-4f;
+__device__ __forceinline__ float beta_term(float a) {
+  return logf(a + kBetaEps) - logf(1.f - a + kBetaEps);
+}
+__device__ __forceinline__ float beta_dloss(float a) {
+  return 1.f / (a + kBetaEps) + 1.f / (1.f - a + kBetaEps);
+}
+
 // One group of the reverse sweep: given the group's clipped values, transmittance and Qn
 // (the suffix value just behind the group), emit dL/dvalue per sample, scatter it, and
 // return in Qn the suffix value in front of the group.
 template <int INTERP>
 __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, const float4& v, uint32_t clip,
-                                              float Ti, const float4& g, float4& Qn,
-                                              float* grad, uint32_t* s_cell, float4* s_val) {
+                                              float Ti, const float4& g, float beta_scale,
+                                              float4& Qn, float* grad, uint32_t* s_cell,
+                                              float4* s_val) {
   const float a = keep ? v.w : 0.f;
   Affine4 f;
   f.a0 = a * v.x;
@@ -151,6 +163,7 @@ __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, in
     gv.y = w * g.y;
     gv.z = w * g.z;
     gv.w = Ti * (g.x * (v.x - s0) + g.y * (v.y - s1) + g.z * (v.z - s2) + g.w * (1.f - s3));
+    if (beta_scale != 0.f) gv.w += beta_scale * beta_dloss(v.w);
   }
   scatter_chunk<INTERP>(P, R, lane, k, keep, clip, gv, grad, s_cell, s_val);
   Qn = make_float4(readlane(f.a0, 0), readlane(f.a1, 0), readlane(f.a2, 0), readlane(f.a3, 0));
@@ -213,18 +226,25 @@ __global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* _
     if (valid) eval_kept<DT, INTERP>(P, R, k, v, clip);
     const float incl = wave_prefix_mul(1.f - v.w);
     const float Ti = T0 * wave_shift_up(incl, 1.f);
-    reverse_chunk<INTERP>(P, R, lane, k, valid, v, clip, Ti, g, Qn, grad, L::cells(mem), L::vals(mem));
+    reverse_chunk<INTERP>(P, R, lane, k, valid, v, clip, Ti, g, 0.f, Qn, grad, L::cells(mem), L::vals(mem));
   }
 }
 
 // Training step for S <= 64*NG: every kept sample's value stays in registers between the
 // forward sweep and the reverse sweep, the loss gradient is formed in place -- one gather of
 // each kept sample per step.
+struct LossParams {
+  const float* targets;
+  float grad_scale, beta_scale;
+  float* sq_err;
+  float* beta;
+};
+
 template <int DT, int INTERP, int NG>
-__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* __restrict__ targets,
-                                                        float grad_scale, float* __restrict__ out,
-                                                        float* __restrict__ sq_err,
+__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
+                                                        float* __restrict__ out,
                                                         float* __restrict__ grad) {
+  const float* targets = LP.targets;
   using L = WaveLds<INTERP, kWave * NG>;
   __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
   const int lane = threadIdx.x & 63;
@@ -245,9 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* _
   float TI[NG];
   int KI[NG];
   uint64_t KEEP[NG];
-  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f, bsum = 0.f;
   int n_proc = S;
   bool stopped = false;
+  const float f0 = beta_term(0.f);
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     V[g] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -265,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* _
       bool proc = valid;
       stopped = stop_check(P, valid, Ti, k, lane, proc, n_proc);
       accumulate(proc, Ti, V[g], cr, cg, cb, ca);
+      if (LP.beta && proc) bsum += beta_term(V[g].w) - f0;
       TI[g] = Ti;
       KI[g] = k;
       KEEP[g] = __ballot(proc);
@@ -276,20 +298,23 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, const float* _
   cg = wave_sum(cg);
   cb = wave_sum(cb);
   ca = wave_sum(ca);
+  if (LP.beta) bsum = wave_sum(bsum);
   const float d0 = cr - tgt.x, d1 = cg - tgt.y, d2 = cb - tgt.z, d3 = ca - tgt.w;
   if (lane == 0) {
     if (out) reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
-    if (sq_err) sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+    if (LP.sq_err) LP.sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+    if (LP.beta) LP.beta[ray] = bsum;
   }
-  const float4 gl = make_float4(grad_scale * d0, grad_scale * d1, grad_scale * d2, grad_scale * d3);
+  const float gs = LP.grad_scale;
+  const float4 gl = make_float4(gs * d0, gs * d1, gs * d2, gs * d3);
   float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int g = NG - 1; g >= 0; --g) {
     // a group with no processed sample is the identity map: Qn passes through unchanged
     if (g < ng && KEEP[g] != 0ull) {
       const bool keep = (KEEP[g] >> lane) & 1ull;
-      reverse_chunk<INTERP>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, Qn, grad,
-                            L::cells(mem), L::vals(mem));
+      reverse_chunk<INTERP>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, LP.beta_scale, Qn,
+                            grad, L::cells(mem), L::vals(mem));
     }
   }
 }
@@ -373,9 +398,13 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
 
 // ============================================================================= host ABI
 
-namespace {
+namespace pxl_host {
 
+namespace {
 thread_local std::string g_err;
+}
+
+const char* last_error_text() { return g_err.c_str(); }
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -393,8 +422,19 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(PXL_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
-constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // 2^24 blocks of 4 rays
+int launched(const char* what) {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PXL_OK : hip_fail(e, what);
+}
+
+int check_dims(int64_t X, int64_t Y, int64_t Z) {
+  if (X <= 0 || Y <= 0 || Z <= 0)
+    return fail(PXL_E_INVALID, "grid dims must be positive (%lld, %lld, %lld)", (long long)X,
+                (long long)Y, (long long)Z);
+  if (X > kMaxCells || Y > kMaxCells || Z > kMaxCells || X * Y > kMaxCells || X * Y * Z > kMaxCells)
+    return fail(PXL_E_SIZE, "grid has more than 2^31-1 cells");
+  return PXL_OK;
+}
 
 int check_grid(const pxl_grid* g) {
   if (!g) return fail(PXL_E_INVALID, "grid descriptor is NULL");
@@ -402,13 +442,19 @@ int check_grid(const pxl_grid* g) {
   if (g->channels != 4) return fail(PXL_E_DTYPE, "grid must have 4 channels, got %d", g->channels);
   if (g->dtype != PXL_DTYPE_F32 && g->dtype != PXL_DTYPE_BF16 && g->dtype != PXL_DTYPE_F16)
     return fail(PXL_E_DTYPE, "unsupported grid dtype %d", g->dtype);
-  for (int a = 0; a < 3; ++a)
-    if (g->dims[a] <= 0) return fail(PXL_E_INVALID, "grid dim %d is %lld", a, (long long)g->dims[a]);
-  if (g->dims[0] > kMaxCells || g->dims[1] > kMaxCells || g->dims[2] > kMaxCells ||
-      g->dims[0] * g->dims[1] > kMaxCells || g->dims[0] * g->dims[1] * g->dims[2] > kMaxCells)
-    return fail(PXL_E_SIZE, "grid has more than 2^31-1 cells");
-  return PXL_OK;
+  return check_dims(g->dims[0], g->dims[1], g->dims[2]);
 }
+
+}  // namespace pxl_host
+
+namespace {
+
+using pxl_host::check_grid;
+using pxl_host::fail;
+using pxl_host::hip_fail;
+using pxl_host::kMaxCells;
+
+constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // 2^24 blocks of 4 rays
 
 int check_march(const pxl_march* m) {
   if (!m) return fail(PXL_E_INVALID, "march descriptor is NULL");
@@ -533,7 +579,7 @@ extern "C" {
 
 int pxl_abi_version(void) { return PXL_ABI_VERSION; }
 
-const char* pxl_last_error(void) { return g_err.c_str(); }
+const char* pxl_last_error(void) { return pxl_host::last_error_text(); }
 
 int64_t pxl_aux_floats(int32_t n_samples) {
   return 2 + 2 * (n_samples > 0 ? ((int64_t)n_samples + 63) / 64 : 0);
@@ -574,35 +620,39 @@ int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
 }
 
 int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
-                   const float* targets, float grad_scale, float* out_rgba, float* ray_sq_err,
-                   float* grad_grid, void* stream) {
+                   const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream) {
   int rc;
+  if (!loss) return fail(PXL_E_INVALID, "loss descriptor is NULL");
   if ((rc = check_grid(grid)) || (rc = check_march(march)) ||
-      (rc = check_rays(rays, targets == nullptr)))
+      (rc = check_rays(rays, loss->targets == nullptr)))
     return rc;
   if (!grad_grid) return fail(PXL_E_INVALID, "grad_grid is NULL");
-  if (!std::isfinite(grad_scale)) return fail(PXL_E_INVALID, "grad_scale must be finite");
+  if (!std::isfinite(loss->grad_scale) || !std::isfinite(loss->beta_scale))
+    return fail(PXL_E_INVALID, "grad_scale and beta_scale must be finite");
   if (march->n_samples > 512)
     return fail(PXL_E_INVALID, "pxl_train_step handles n_samples <= 512 (got %d); use "
                 "pxl_render_fwd + pxl_render_bwd", march->n_samples);
   if (rays->n_rays == 0) return PXL_OK;
   pxl::KParams P = make_params(grid, march, rays);
+  pxl::LossParams LP{loss->targets, loss->grad_scale, loss->beta_scale, loss->ray_sq_err,
+                     loss->ray_beta};
   hipStream_t s = (hipStream_t)stream;
   const int S = march->n_samples;
   PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
     rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+      const pxl::LossParams LQ = LP;  // per-ray arrays are indexed by the global ray number
       if (S <= 64)
         hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 1>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
-                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+                           LQ, out_rgba, grad_grid);
       else if (S <= 128)
         hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 2>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
-                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+                           LQ, out_rgba, grad_grid);
       else if (S <= 256)
         hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 4>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
-                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+                           LQ, out_rgba, grad_grid);
       else
         hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 8>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
-                           targets, grad_scale, out_rgba, ray_sq_err, grad_grid);
+                           LQ, out_rgba, grad_grid);
     }, "pxl_train_step");
   });
   return rc;

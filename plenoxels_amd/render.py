@@ -206,14 +206,17 @@ def camera_rays(camera, want_targets=True):
 
 
 def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, targets=None,
-               n_total_rays=None, occupancy=None, out=None, ray_sq_err=None):
-    """Render + mean-MSE gradient + backward in one kernel pass.
+               n_total_rays=None, occupancy=None, out=None, ray_sq_err=None, beta=0.0,
+               n_total_samples=None, ray_beta=None):
+    """Render + loss gradient + backward in one kernel pass (scripts/train.py:130-181 minus
+    the optimiser and the TV term).
 
-    Accumulates d mse_loss / d grid into grad_grid ((X,Y,Z,4) float32; zero it first for a
-    plain gradient).  Rays are explicit (origins, dirs) or a CameraRays; targets (R,4), or
-    None to gather them from camera.imgs.  n_total_rays: the loss normaliser 4*N of the full
-    batch this call is part of (default: this call's rays).  Returns ray_sq_err (R,) if
-    requested or allocated: loss = ray_sq_err.sum() / (4 * n_total_rays).
+    Accumulates d loss / d grid into grad_grid ((X,Y,Z,4) float32; zero it first for a plain
+    gradient).  loss = mse_loss(out, targets) + beta * mean_samples(log(a+1e-4) -
+    log(1-a+1e-4)).  Rays are explicit (origins, dirs) or a CameraRays; targets (R,4), or
+    None to gather them from camera.imgs.  n_total_rays / n_total_samples: the normalisers
+    of the full batch this call is part of (default: this call's).  ray_sq_err (R,) and
+    ray_beta (R,) receive per-ray loss partials when given (see pxl_loss in pxl.h).
     """
     L = _lib.lib()
     if grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != tuple(grid.shape) or not grad_grid.is_contiguous():
@@ -229,15 +232,31 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
     if targets is not None:
         if targets.dtype != torch.float32 or tuple(targets.shape) != (R, 4) or not targets.is_contiguous():
             raise ValueError("targets must be contiguous (R,4) float32")
-    n_total = R if n_total_rays is None else int(n_total_rays)
-    scale = 2.0 / (4.0 * n_total)
+    for name, t in (("out", out), ("ray_sq_err", ray_sq_err), ("ray_beta", ray_beta)):
+        if t is not None and (t.dtype != torch.float32 or not t.is_contiguous() or
+                              t.numel() != (4 * R if name == "out" else R)):
+            raise ValueError(f"{name} has the wrong shape or dtype")
+    n_rays_total = R if n_total_rays is None else int(n_total_rays)
+    n_samples_total = (R * cfg.num_samples) if n_total_samples is None else int(n_total_samples)
+    loss = _lib.Loss()
+    loss.targets = None if targets is None else targets.data_ptr()
+    loss.grad_scale = 2.0 / (4.0 * n_rays_total)
+    loss.beta_scale = float(beta) / n_samples_total if beta else 0.0
+    loss.ray_sq_err = None if ray_sq_err is None else ray_sq_err.data_ptr()
+    loss.ray_beta = None if ray_beta is None else ray_beta.data_ptr()
     g = _grid_struct(grid, occupancy)
     m = cfg.to_struct()
-    _lib.check(L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
-                                _lib.ptr(targets), ctypes.c_float(scale), _lib.ptr(out),
-                                _lib.ptr(ray_sq_err), _lib.ptr(grad_grid),
-                                _lib.stream_handle(grid.device)))
+    _lib.check(L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
+                                _lib.ptr(out), _lib.ptr(grad_grid), _lib.stream_handle(grid.device)))
     return ray_sq_err
+
+
+BETA_EPS = 1e-4
+
+
+def beta_term_zero():
+    """f(0) of the per-sample beta term f(a) = log(a + 1e-4) - log(1 - a + 1e-4)."""
+    return float(np.log(np.float32(BETA_EPS)) - np.log(np.float32(1.0 + BETA_EPS)))
 
 
 def fill_(t, value=0.0):
@@ -268,4 +287,18 @@ def pack_occupancy(occ):
     out = torch.empty((n + 31) // 32, dtype=torch.int32, device=occ.device)
     _lib.check(_lib.lib().pxl_pack_occupancy(_lib.ptr(src), n, _lib.ptr(out),
                                              _lib.stream_handle(occ.device)))
+    return out
+
+
+def render_camera(grid, camera, cfg, occupancy=None):
+    """Forward-only render of camera-form rays: (R,4) RGBA (the chain of
+    src/visualization.py:125-146 in one kernel)."""
+    camera.validate()
+    L = _lib.lib()
+    r = _rays_struct(camera=camera)
+    out = torch.empty((r.n_rays, 4), dtype=torch.float32, device=grid.device)
+    g = _grid_struct(grid, occupancy)
+    m = cfg.to_struct()
+    _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), _lib.ptr(out), None,
+                                _lib.stream_handle(grid.device)))
     return out

@@ -39,6 +39,7 @@ def test_struct_layout_matches_header(tmp_path):
                       "inv_pd", "grid_min"],
         "pxl_rays": ["n_rays", "origins", "dirs", "cam_ids", "uv", "c2w", "n_cams",
                      "camera_angle_x", "reserved", "imgs", "img_h", "img_w"],
+        "pxl_loss": ["targets", "grad_scale", "beta_scale", "ray_sq_err", "ray_beta"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pxl.h"', "int main(void){"]
     for s, fs in fields.items():
@@ -51,7 +52,7 @@ def test_struct_layout_matches_header(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(probe), "-o", str(exe)], check=True)
     got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
                                                  check=True).stdout.splitlines())
-    mirrors = {"pxl_grid": _lib.Grid, "pxl_march": _lib.March, "pxl_rays": _lib.Rays}
+    mirrors = {"pxl_grid": _lib.Grid, "pxl_march": _lib.March, "pxl_rays": _lib.Rays, "pxl_loss": _lib.Loss}
     for s, cls in mirrors.items():
         assert int(got[s]) == ctypes.sizeof(cls), s
         for f in fields[s]:
@@ -123,9 +124,27 @@ def test_validation_errors_before_any_hip_call():
     g2, m2, r2 = _valid()
     m2.n_samples = 600
     r2.n_rays = 4
-    rc = L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), None, ctypes.c_float(1.0),
-                          None, None, ctypes.c_void_p(0x5000), None)
+    loss = _lib.Loss()
+    loss.grad_scale = 1.0
+    rc = L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), ctypes.byref(loss),
+                          None, ctypes.c_void_p(0x5000), None)
     assert rc == _lib.PXL_E_INVALID  # targets missing for explicit rays
+    loss.targets = 0x6000
+    rc = L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), ctypes.byref(loss),
+                          None, ctypes.c_void_p(0x5000), None)
+    assert rc == _lib.PXL_E_INVALID and "512" in _lib.last_error()  # fused step: S <= 512
+    assert L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), None, None,
+                            ctypes.c_void_p(0x5000), None) == _lib.PXL_E_INVALID
+    d3 = _lib.dims_arg((4, 4, 0))
+    assert L.pxl_wrap_indices(ctypes.c_void_p(0x10), 4, d3, None) == _lib.PXL_E_INVALID
+    assert L.pxl_avg_pool3d(ctypes.c_void_p(0x10), 4, 4, 4, 5, 1, ctypes.c_void_p(0x20), None) == _lib.PXL_E_INVALID
+    assert L.pxl_adam_step(ctypes.c_void_p(0x10), ctypes.c_void_p(0x10), ctypes.c_void_p(0x10),
+                           ctypes.c_void_p(0x10), None, 4, 1e-3, 0.9, 0.999, 1e-8, 0, None, 0,
+                           None) == _lib.PXL_E_INVALID  # step counts from 1
+    g3, _, _ = _valid()
+    g3.dtype = _lib.DTYPE_BF16
+    assert L.pxl_nearest_gather(ctypes.c_void_p(0x10), 4, ctypes.byref(g3), ctypes.c_void_p(0x20), None,
+                                None) == _lib.PXL_E_DTYPE
     assert L.pxl_fill_f32(None, 4, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_fill_f32(ctypes.c_void_p(0x1004), 8, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_occupancy_dilate(ctypes.c_void_p(0x10), 4, 4, 4, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID

@@ -1,0 +1,290 @@
+"""GPU parity of the per-function drop-ins (plenoxels_amd.ray_sampling / grid_functions /
+optim / train) against the oracle on the same inputs.
+
+Bit-exact where the reference's arithmetic is plain float32 elementwise work (sample
+positions, normalisation, voxel choice, corner indices, gathers, lerp order, coordinates);
+RGBA within 1e-5 absolute; gradients within 1e-4 relative (of the largest element, and per
+element above 1% of it); Adam within a few float32 ulps of torch.optim.Adam.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import plain
+from plenoxels_amd import grid_functions as gf
+from plenoxels_amd import ray_sampling as rs
+from plenoxels_amd.optim import Adam, tv_loss
+from util import grad_close, random_cameras, say, small_scene
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _cams(B=3, H=16, seed=5):
+    c2w = random_cameras(B, seed, radius=2.2, y_scale=1.2)
+    imgs = torch.rand(B, H, H, 4, generator=torch.Generator().manual_seed(seed + 1))
+    return c2w, imgs
+
+
+def test_normalize_samples_bitwise():
+    say("normalize_samples_for_indecies")
+    gi, _, _, _ = plain.generate_grid(20, 21, 22, points_distance=0.0125)
+    p = torch.rand(5000, 3, generator=torch.Generator().manual_seed(1)) * 0.6 - 0.3
+    want = plain.normalize_samples(p, plain.grid_min(gi), 0.0125, "reciprocal")
+    got = rs.normalize_samples_for_indecies(gi.to(DEV), p.to(DEV), 0.0125)
+    assert torch.equal(got.cpu(), want)
+
+
+@pytest.mark.parametrize("even", [False, True])
+def test_sample_camera_rays_batched_bitwise(even):
+    say("sample_camera_rays_batched even_spread", even)
+    c2w, imgs = _cams()
+    R, S, delta = 49, 7, 0.21
+    torch.manual_seed(123)
+    uv = torch.rand(3, R, 2, device=DEV).cpu() if not even else plain.even_spread_uniforms(R, 3)
+    torch.manual_seed(123)
+    smp, pix, cpos, dirs = rs.sample_camera_rays_batched(c2w.to(DEV), 0.69, imgs.to(DEV), R, S, delta,
+                                                         even_spread=even, camera_ray=False, device="cuda")
+    d_w, pix_w = plain.generate_rays_batched(imgs, R, c2w, 0.69, uv)
+    o_w = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
+    smp_w = plain.sample_positions(o_w, d_w, S, delta)
+    assert torch.equal(dirs.cpu(), d_w)
+    assert torch.equal(pix.cpu(), pix_w)
+    assert torch.equal(smp.cpu(), smp_w)
+    assert torch.equal(cpos.cpu(), c2w[:, :3, 3])
+    with pytest.raises(NotImplementedError):
+        rs.sample_camera_rays_batched(c2w.to(DEV), 0.69, imgs.to(DEV), R, S, delta, False, True)
+
+
+def _points(seed, X=9, N=3000):
+    g = torch.Generator().manual_seed(seed)
+    n = torch.rand(N, 3, generator=g) * (X + 3) - 1.5
+    ties = torch.tensor([[0.5, 1.5, 2.5], [-0.5, 8.5, 7.5], [8.49, -0.51, 3.0], [9.0, 0.0, 4.5]])
+    return torch.cat([ties, n], 0)
+
+
+def test_bounds_wrap_bitwise():
+    say("find_out_of_bound / fix_out_of_bounds")
+    grid = torch.zeros(9, 10, 11, 4)
+    n = _points(2, X=10)
+    idx = torch.round(n).long()
+    assert torch.equal(gf.find_out_of_bound(idx.to(DEV), grid).cpu(), plain.find_out_of_bound(idx, (9, 10, 11)))
+    assert torch.equal(gf.find_out_of_bound(n.to(DEV), grid).cpu(), plain.find_out_of_bound(n, (9, 10, 11)))
+    w = idx.to(DEV)
+    a, b, c = gf.fix_out_of_bounds(w, grid)
+    assert torch.equal(w.cpu(), plain.wrap_indices(idx, (9, 10, 11)))
+    assert torch.equal(a.cpu(), w[:, 0].cpu())
+
+
+def test_nearest_voxels_fwd_bwd():
+    say("get_nearest_voxels")
+    g = torch.Generator().manual_seed(3)
+    grid = torch.rand(9, 9, 9, 4, generator=g)
+    n = _points(4)
+    gd = grid.to(DEV).requires_grad_(True)
+    vals, mask = gf.get_nearest_voxels(n.to(DEV), gd)
+    idx = torch.round(n).long()
+    m_w = plain.find_out_of_bound(idx, (9, 9, 9))
+    w = plain.wrap_indices(idx, (9, 9, 9))
+    go = grid.clone().requires_grad_(True)
+    v_w = go[w[:, 0], w[:, 1], w[:, 2]]
+    assert torch.equal(mask.cpu(), m_w)
+    assert torch.equal(vals.detach().cpu(), v_w.detach())
+    up = torch.randn(vals.shape, generator=g)
+    (vals * up.to(DEV)).sum().backward()
+    (v_w * up).sum().backward()
+    ok, info = grad_close(gd.grad.cpu(), go.grad)
+    assert ok, info
+
+
+def test_corners_collect_trilinear():
+    say("corners / collect / trilinear")
+    g = torch.Generator().manual_seed(6)
+    X = 9
+    grid = torch.rand(X, X, X, 4, generator=g) * 1.2 - 0.1
+    n = _points(7, X=X)
+    c = gf.get_grid_points_indices(n.to(DEV))
+    assert torch.equal(c.cpu(), plain.get_grid_points_indices(n))
+    gd = grid.to(DEV).requires_grad_(True)
+    cells, mask = gf.collect_cell_information_via_indices(n.to(DEV), gd)
+    go = grid.clone().requires_grad_(True)
+    cells_w, mask_w = plain.collect_cell_information_via_indices(n, go)
+    assert torch.equal(mask.cpu(), mask_w)
+    assert torch.equal(cells.detach().cpu(), cells_w.detach())
+    up = torch.randn(cells.shape, generator=g)
+    (cells * up.to(DEV)).sum().backward()
+    (cells_w * up).sum().backward()
+    assert grad_close(gd.grad.cpu(), go.grad)[0]
+    # trilinear on wrapped corners (what the reference's callers pass)
+    cw = plain.wrap_indices(plain.get_grid_points_indices(n).reshape(-1, 3), (X, X, X)).reshape(-1, 8, 3)
+    gd2 = grid.to(DEV).requires_grad_(True)
+    tri = gf.trilinear_interpolation(n.to(DEV), cw.to(DEV), gd2)
+    go2 = grid.clone().requires_grad_(True)
+    tri_w = plain.trilinear_interpolation(n, cw, go2)
+    assert torch.equal(tri.detach().cpu(), tri_w.detach())
+    up = torch.randn(tri.shape, generator=g)
+    (tri * up.to(DEV)).sum().backward()
+    (tri_w * up).sum().backward()
+    ok, info = grad_close(gd2.grad.cpu(), go2.grad)
+    assert ok, info
+
+
+@pytest.mark.parametrize("S", [1, 64, 100, 600])
+def test_compute_alpha_weighted_pixels_fwd_bwd(S):
+    say("compute_alpha_weighted_pixels S", S)
+    g = torch.Generator().manual_seed(8 + S)
+    x = torch.rand(2, 5, S, 4, generator=g)
+    x[..., 3] *= 0.2
+    x[0, 1, S // 2, 3] = 1.0
+    xd = x.to(DEV).requires_grad_(True)
+    out = rs.compute_alpha_weighted_pixels(xd)
+    xo = x.double().requires_grad_(True)
+    want = plain.compute_alpha_weighted_pixels(xo)
+    assert out.shape == (2, 5, 4)
+    assert (out.detach().cpu().double() - want.detach()).abs().max().item() <= 1e-5
+    up = torch.randn(out.shape, generator=g)
+    (out * up.to(DEV)).sum().backward()
+    (want * up.double()).sum().backward()
+    ok, info = grad_close(xd.grad.cpu(), xo.grad)
+    assert ok, info
+
+
+@pytest.mark.parametrize("k,s", [(3, 1), (5, 2), (13, 3), (33, 8)])
+def test_average_pool3d_grid_fwd_bwd(k, s):
+    say("average_pool3d_grid", k, s)
+    g = torch.Generator().manual_seed(k)
+    # every pooled axis > 1: the reference's .squeeze() (src/grid_functions.py:180) breaks otherwise
+    grid = torch.rand(40, 37, 35, 4, generator=g) if k < 20 else torch.rand(64, 60, 70, 4, generator=g)
+    gd = grid.to(DEV).requires_grad_(True)
+    out = gf.average_pool3d_grid(gd, receptive_field_size=k, stride=s)
+    go = grid.double().requires_grad_(True)
+    want = plain.average_pool3d_grid(go, k, s)
+    assert out.shape == want.shape
+    assert (out.detach().cpu().double() - want.detach()).abs().max().item() <= 1e-6
+    up = torch.randn(out.shape, generator=g)
+    (out * up.to(DEV)).sum().backward()
+    (want * up.double()).sum().backward()
+    ok, info = grad_close(gd.grad.cpu(), go.grad)
+    assert ok, info
+
+
+def test_generate_grid_bitwise():
+    say("generate_grid")
+    for dims, pd in (((5, 6, 7), 0.25), ((64, 63, 2), 3 / 63)):
+        want = plain.generate_grid(*dims, points_distance=pd)
+        got = gf.generate_grid(*dims, points_distance=pd, device="cuda")
+        for a, b in zip(got, want):
+            assert torch.equal(a.detach().cpu(), b.detach())
+        assert got[1].requires_grad
+
+
+def test_tv_loss_fwd_bwd():
+    say("tv_loss")
+    g = torch.Generator().manual_seed(12)
+    grid = torch.rand(17, 19, 21, 4, generator=g)
+    gd = grid.to(DEV).requires_grad_(True)
+    v = tv_loss(gd)
+    go = grid.double().requires_grad_(True)
+    w = plain.tv_loss(go)
+    assert abs(float(v.detach()) - float(w.detach())) <= 1e-6 * float(w.detach())
+    (3.0 * v).backward()
+    (3.0 * w).backward()
+    ok, info = grad_close(gd.grad.cpu(), go.grad)
+    assert ok, info
+
+
+def test_adam_matches_torch():
+    say("Adam")
+    g = torch.Generator().manual_seed(13)
+    p0 = torch.randn(6, 7, 8, 4, generator=g)
+    grads = [torch.randn(6, 7, 8, 4, generator=g) * 10 ** (-i) for i in range(4)]
+    pd = p0.clone().to(DEV).requires_grad_(True)
+    gabs = torch.zeros_like(pd)
+    half = torch.empty_like(pd, dtype=torch.bfloat16)
+    opt = Adam([pd], lr=0.0075, grad_abs={pd: gabs}, half_copy={pd: half})
+    pr = p0.clone().requires_grad_(True)
+    ref = torch.optim.Adam([pr], lr=0.0075)
+    for gr in grads:
+        pd.grad = gr.to(DEV)
+        opt.step()
+        pr.grad = gr.clone()
+        ref.step()
+    assert torch.allclose(pd.detach().cpu(), pr.detach(), rtol=1e-6, atol=1e-6)
+    st = opt.state[pd]
+    for key in ("exp_avg", "exp_avg_sq"):  # within a rounding step of the largest moment
+        a, b = st[key].cpu(), ref.state[pr][key]
+        assert ((a - b).abs() <= 1e-6 * b.abs() + 1e-7 * b.abs().max()).all(), key
+    assert torch.allclose(gabs.cpu(), sum(x.abs() for x in grads), rtol=1e-6)
+    assert torch.equal(half.cpu(), pd.detach().cpu().to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_train_step_beta_term(interp):
+    """The beta regulariser of scripts/train.py:170-177 fused into the train step."""
+    from plenoxels_amd import MarchConfig, train_step
+    from plenoxels_amd.render import beta_term_zero
+    say("beta term", interp)
+    grid, o, d, gmin, pd = small_scene(90)
+    R, S, beta = o.shape[0], 64, 5e-3
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(91))
+    kw = dict(num_samples=S, delta_step=0.06, points_distance=pd)
+    go = grid.clone().requires_grad_(True)
+    p = plain.sample_positions(o, d, S, 0.06)
+    n = plain.normalize_samples(p, gmin, pd, "reciprocal")
+    vals, _ = (plain.nearest_values if interp == "nearest" else plain.trilinear_values)(n, go)
+    vals = vals.reshape(R, S, 4)
+    out_w = plain.compute_alpha_weighted_pixels(vals)
+    a = vals[..., -1]
+    bl = beta * (torch.log(a + 1e-4) - torch.log(1 - a + 1e-4)).mean()
+    (plain.mse_loss(out_w, tgt) + bl).backward()
+    gg = torch.zeros(grid.shape, device=DEV)
+    rb = torch.empty(R, device=DEV)
+    train_step(grid.to(DEV), gg, cfg=MarchConfig(S, 0.06, tuple(gmin.tolist()), pd, interp), origins=o.to(DEV),
+               dirs=d.to(DEV), targets=tgt.to(DEV), beta=beta, ray_beta=rb)
+    bl_k = beta * (rb.sum().item() + R * S * beta_term_zero()) / (R * S)
+    assert abs(bl_k - float(bl.detach())) <= 1e-5 * abs(float(bl.detach()))
+    ok, info = grad_close(gg.cpu(), go.grad)
+    say("  grad", info)
+    assert ok, info
+
+
+def _fit_inputs(B=3, H=24, R=64, steps=4, seed=20):
+    c2w = random_cameras(B, seed, radius=2.0)
+    imgs = torch.rand(B, H, H, 4, generator=torch.Generator().manual_seed(seed + 1))
+    g = torch.Generator().manual_seed(seed + 2)
+    uniforms = [torch.rand(B, R, 2, generator=g) for _ in range(steps)]
+    return c2w, imgs, uniforms
+
+
+@pytest.mark.parametrize("start,dims,progressive", [(0, (24, 24, 24), False), (205, (48, 48, 48), True)])
+def test_trainer_matches_reference_loop(start, dims, progressive):
+    """Trainer.step (fused) vs the reference loop restated op for op (oracle.plain.reference_fit),
+    same uniforms: full-grid steps, and progressive-growing steps with k = 13..11 pooling."""
+    from plenoxels_amd.train import Trainer
+    say("trainer vs reference loop", start, dims)
+    c2w, imgs, uniforms = _fit_inputs()
+    # a non-constant start grid: from all zeros the first TV step (k < 19) divides by
+    # sqrt(0) and both loops turn NaN, as the reference does
+    init = torch.rand(*dims, 4, generator=torch.Generator().manual_seed(30)) * 0.6 - 0.1
+    kw = dict(gridsize=dims, points_distance=0.06, num_samples=48, delta_step=0.05, lr=0.01, tv=1e-3,
+              beta=5e-3, init_grid=init)
+    grid_w, gabs_w, hist_w = plain.reference_fit(imgs, c2w, 0.69, uniforms=uniforms, div_mode="reciprocal",
+                                                 progressive_growing=progressive, start_step=start, **kw)
+    kw["init_grid"] = init.to(DEV)
+    tr = Trainer(imgs, c2w, 0.69, number_of_rays=64, device="cuda", progressive_growing=progressive,
+                 uniforms_fn=lambda i: uniforms[i - start], **kw)
+    for i in range(start, start + len(uniforms)):
+        out = tr.step(i)
+        rec = hist_w[i - start]
+        assert set(out) - {"loss"} == set(rec), (i, sorted(out), sorted(rec))
+        for key, v in rec.items():
+            assert abs(float(out[key]) - v) <= 1e-4 * abs(v) + 1e-9, (i, key, float(out[key]), v)
+    ok, info = grad_close(tr.grad_abs.cpu(), gabs_w, rel=1e-3)
+    say("  |grad| sum", info)
+    assert ok, info
+    diff = (tr.grid.cpu() - grid_w).abs()
+    say("  grid: max diff", diff.max().item(), "frac > 1e-5:", (diff > 1e-5).float().mean().item())
+    assert (diff > 1e-5).float().mean().item() <= 1e-3
+    assert diff.max().item() <= 2 * 0.01 * len(uniforms)

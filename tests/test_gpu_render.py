@@ -272,3 +272,30 @@ def test_sparse256_subset_train_step_vs_plain(t_stop):
         assert ok
     else:
         assert (err <= 1e-5).float().mean().item() >= 0.995
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_nan_cell_propagates_like_torch_clamp(interp):
+    """torch.clamp keeps NaN, so a NaN cell read by a ray's in-grid sample makes that pixel
+    NaN in the reference chain; the kernels must not hide it.  (The reference also gathers the
+    wrapped cell of an out-of-grid sample and multiplies it by 0, so NaN * 0 can reach a pixel
+    through a sample outside the grid; the kernels skip those samples.)"""
+    say("nan propagation", interp)
+    grid, o, d, gmin, pd = small_scene(100)
+    grid[10:14, 10:14, :, 3] = float("nan")
+    S, delta = 96, 0.04
+    kw = dict(num_samples=S, delta_step=delta, points_distance=pd, interp=interp)
+    want = plain.render(grid, o, d, gmin=gmin, div_mode="reciprocal", **kw)
+    got = render_rays(grid.to(DEV), o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), **kw).cpu()
+    # in-grid reads only: zero the out-of-grid samples before compositing
+    n = plain.normalize_samples(plain.sample_positions(o, d, S, delta), gmin, pd, "reciprocal")
+    vals, mask = (plain.nearest_values if interp == "nearest" else plain.trilinear_values)(n, grid)
+    vals = torch.where(mask.unsqueeze(-1), vals, torch.zeros_like(vals))
+    want_in = plain.compute_alpha_weighted_pixels(vals.reshape(-1, S, 4))
+    nan_in = torch.isnan(want_in).any(1)
+    nan_got = torch.isnan(got).any(1)
+    assert nan_in.any() and not nan_in.all()
+    assert torch.equal(nan_got, nan_in)
+    assert bool((nan_got <= torch.isnan(want).any(1)).all())
+    ok = ~torch.isnan(want).any(1)
+    assert (got[ok] - want[ok]).abs().max().item() <= 1e-5

@@ -390,3 +390,24 @@ def test_c_oracle_central_differences():
         lm = ((cref.render(gm.reshape(g.shape), o.numpy(), d.numpy(), **kw)[0] - tgt) ** 2).mean()
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - gg.reshape(-1)[L]) <= 1e-6 * max(1e-3, abs(fd))
+
+
+def test_golden_pool_and_tv(golden):
+    x = T(golden["pool_in"])
+    assert torch.allclose(plain.average_pool3d_grid(x, 5, 1), T(golden["pool_k5_s1"]), atol=1e-6)
+    assert torch.allclose(plain.average_pool3d_grid(x, 7, 2), T(golden["pool_k7_s2"]), atol=1e-6)
+    t = T(golden["tv_in"]).clone().requires_grad_(True)
+    v = plain.tv_loss(t)
+    v.backward()
+    assert abs(float(v.detach()) - float(golden["tv_out"])) < 1e-5
+    assert torch.allclose(t.grad, T(golden["tv_grad"]), atol=1e-7)
+
+
+def test_tv_loss_closed_form():
+    g = torch.zeros(3, 4, 5, 4, dtype=torch.float64)
+    g[1, 2, 3, 0] = 2.0  # one spike: 6 neighbours (all inside) each contribute 2^2
+    assert float(plain.tv_loss(g)) == pytest.approx(math.sqrt(6 * 4.0))
+    g = torch.arange(3 * 4 * 5 * 4, dtype=torch.float64).reshape(3, 4, 5, 4)
+    # linear ramp: differences are the strides 4 (z), 20 (y), 80 (x)
+    want = math.sqrt(4 * (3 * 4 * 4 * 16 + 3 * 3 * 5 * 400 + 2 * 4 * 5 * 6400))
+    assert float(plain.tv_loss(g)) == pytest.approx(want)
