@@ -333,6 +333,8 @@ __device__ __forceinline__ bool trilinear_floor_cell(const KParams& P, float nx,
 
 // True when sample k lies in the grid and is not skipped by occupancy: the samples whose
 // values the reference chain can make nonzero.  Geometry only -- no grid value is read.
+// Branch-free: every lane loads an occupancy word (word 0 for a sample outside the grid), so
+// the loads of several chunks classified back to back are in flight together.
 template <int INTERP>
 __device__ __forceinline__ bool classify(const KParams& P, const Ray& R, int k) {
   float nx, ny, nz;
@@ -340,7 +342,9 @@ __device__ __forceinline__ bool classify(const KParams& P, const Ray& R, int k) 
   uint32_t cell;
   const bool inb = INTERP == PXL_INTERP_NEAREST ? nearest_cell(P, nx, ny, nz, cell)
                                                 : trilinear_floor_cell(P, nx, ny, nz, cell);
-  return inb && occupied(P.occ, cell);
+  if (P.occ == nullptr) return inb;
+  const uint32_t word = P.occ[cell >> 5];
+  return inb & (((word >> (cell & 31u)) & 1u) != 0u);
 }
 
 // torch.clamp(v, 0, 1): NaN stays NaN (v_med3 alone would return a bound)
@@ -409,10 +413,14 @@ __device__ __forceinline__ int append_kept(const KParams& P, const Ray& R, int k
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Gradient items per sample, and the LDS staging capacity: the items of kStageSamples kept
+// samples are staged and flushed at a time.  Trilinear stages 32 samples (256 items, 5 KiB
+// per wave) so that LDS leaves room for six waves per SIMD.
 template <int INTERP>
 struct ScatterItems {
   static constexpr int kPerSample = INTERP == PXL_INTERP_NEAREST ? 1 : 8;
-  static constexpr int kPerChunk = kPerSample * kWave;
+  static constexpr int kStageSamples = INTERP == PXL_INTERP_NEAREST ? kWave : 32;
+  static constexpr int kPerChunk = kPerSample * kStageSamples;
 };
 
 // Per-sample gradient items: the cells sample k read and d loss / d cell value for each
@@ -463,21 +471,26 @@ __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, in
                                               bool keep, uint32_t clip, const float4& gv,
                                               float* grad, uint32_t* s_cell, float4* s_val) {
   constexpr int NI = ScatterItems<INTERP>::kPerSample;
+  constexpr int NS = ScatterItems<INTERP>::kStageSamples;
   const uint64_t km = __ballot(keep);
   if (km == 0ull) return;
-  const int n = __popcll(km) * NI;
-  if (keep) write_items<INTERP>(P, R, k, clip, gv, s_cell, s_val, lane_rank(km) * NI);
-  wave_lds_sync();
+  const int nk = __popcll(km);
+  const int rank = lane_rank(km);
   const int ch = lane & 3;
   const float* sv = reinterpret_cast<const float*>(s_val);
-  for (int base = 0; base < n; base += kWave / 4) {
-    const int it = base + (lane >> 2);
-    if (it < n) {
-      const float v = sv[4 * it + ch];
-      if (v != 0.f) atomic_add_f32(grad + (size_t)s_cell[it] * 4u + ch, v);
+  for (int h = 0; h < nk; h += NS) {  // kept samples of rank h .. h+NS-1 per stage
+    if (keep && rank >= h && rank < h + NS) write_items<INTERP>(P, R, k, clip, gv, s_cell, s_val, (rank - h) * NI);
+    wave_lds_sync();
+    const int n = min(nk - h, NS) * NI;
+    for (int base = 0; base < n; base += kWave / 4) {
+      const int it = base + (lane >> 2);
+      if (it < n) {
+        const float v = sv[4 * it + ch];
+        if (v != 0.f) atomic_add_f32(grad + (size_t)s_cell[it] * 4u + ch, v);
+      }
     }
+    wave_lds_sync();
   }
-  wave_lds_sync();
 }
 
 }  // namespace pxl
