@@ -249,19 +249,26 @@ __device__ __forceinline__ float4 camera_target(const KParams& P, const Ray& R) 
   return reinterpret_cast<const float4*>(P.imgs)[pix];
 }
 
+// Both forms produce a whole Ray value that is assigned once: branches that stored to
+// different fields of R last were merged by the compiler into one store through a selected
+// field pointer, which forced the ray into scratch memory.
 __device__ __forceinline__ void setup_ray(const KParams& P, int64_t r, Ray& R) {
+  Ray q;
   if (P.origins != nullptr) {
-    R.ox = P.origins[3 * r];
-    R.oy = P.origins[3 * r + 1];
-    R.oz = P.origins[3 * r + 2];
-    R.dx = P.dirs[3 * r];
-    R.dy = P.dirs[3 * r + 1];
-    R.dz = P.dirs[3 * r + 2];
-    R.cam = 0;
-    R.u = R.v = 0.f;
+    q.ox = P.origins[3 * r];
+    q.oy = P.origins[3 * r + 1];
+    q.oz = P.origins[3 * r + 2];
+    q.dx = P.dirs[3 * r];
+    q.dy = P.dirs[3 * r + 1];
+    q.dz = P.dirs[3 * r + 2];
+    q.cam = 0;
+    q.u = q.v = 0.f;
   } else {
-    camera_ray(P, r, R);
+    camera_ray(P, r, q);
   }
+  R.ox = q.ox; R.oy = q.oy; R.oz = q.oz;
+  R.dx = q.dx; R.dy = q.dy; R.dz = q.dz;
+  R.cam = q.cam; R.u = q.u; R.v = q.v;
 }
 
 // ----------------------------------------------------------------------------- geometry

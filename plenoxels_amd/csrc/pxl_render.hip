@@ -241,13 +241,12 @@ struct LossParams {
 };
 
 // Occupancy target of the training step (waves per SIMD; register budget 512 / waves).  The
-// trilinear kernels with up to four groups run 6 waves (5 for FP16 storage, whose unpacking
-// needs more registers), measured 3-7% faster on sparse256 than the compiler's default of 4;
-// the others take what the compiler allocates.
+// trilinear kernels with up to four groups run 5 waves (the compiler alone allocates for 4
+// with FP16 storage), measured 2-7% faster on sparse256; the others take what the compiler
+// allocates.
 template <int DT, int INTERP, int NG>
 struct TrainWaves {
-  static constexpr int value =
-      (INTERP == PXL_INTERP_TRILINEAR && NG <= 4) ? (DT == PXL_DTYPE_F16 ? 5 : 6) : 1;
+  static constexpr int value = (INTERP == PXL_INTERP_TRILINEAR && NG <= 4) ? 5 : 1;
 };
 
 template <int DT, int INTERP, int NG>
