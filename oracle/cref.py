@@ -92,6 +92,39 @@ def render(grid, origins, dirs, *, grad_rgba=None, occupancy_bits=None, **kw):
     return out, gg, nproc
 
 
+def render_parallel(grid, origins, dirs, *, threads=None, chunk=16384, occupancy_bits=None, progress=None, **kw):
+    """Forward-only render of many rays on `threads` host threads (ref_render keeps no state
+    and ctypes releases the GIL): (R,4) float64 RGBA.  progress(done, total) is called after
+    each chunk."""
+    from concurrent.futures import ThreadPoolExecutor
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64))
+    o = np.ascontiguousarray(np.asarray(origins, dtype=np.float32))
+    d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float32))
+    R = o.shape[0]
+    p = make_params(g.shape[:3], **kw)
+    out = np.zeros((R, 4), np.float64)
+    nproc = np.zeros(R, np.int32)
+    occ = None if occupancy_bits is None else np.ascontiguousarray(
+        np.asarray(occupancy_bits).view(np.uint32))
+    L = lib()
+
+    def work(s):
+        e = min(R, s + chunk)
+        rc = L.ref_render(ctypes.byref(p), _ptr(g), _ptr(occ), _ptr(o[s:e]), _ptr(d[s:e]), e - s,
+                          _ptr(out[s:e]), None, None, _ptr(nproc[s:e]))
+        if rc != 0:
+            raise MemoryError("ref_render failed")
+        return e - s
+
+    done = 0
+    with ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 1)) as ex:
+        for n in ex.map(work, range(0, R, chunk)):
+            done += n
+            if progress is not None:
+                progress(done, R)
+    return out
+
+
 def rays_touching(grid, origins, dirs, query_cells, *, occupancy_bits=None, **kw):
     """(R, Q) uint8: 1 where ray r reads linear cell query_cells[q] (query sorted here)."""
     g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64))

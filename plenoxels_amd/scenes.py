@@ -100,6 +100,56 @@ def sphere_cameras(n, seed=1, hemisphere=False, radius=RADIUS):
     return torch.from_numpy(np.stack(mats))
 
 
+def primitive_scene(n=12, seed=0, lo=-1.2, hi=1.2, rmin=0.1, rmax=0.5):
+    """n seeded coloured primitives inside [lo,hi]^3, alternating spheres (radius ~ U[rmin,rmax])
+    and axis-aligned boxes (half-extents ~ U[rmin,rmax]^3): list of dicts with kind, center (3,),
+    half-size (3,) and RGB colour (3,), all float64."""
+    g = np.random.default_rng(seed + 2000)
+    prims = []
+    for i in range(n):
+        kind = "sphere" if i % 2 == 0 else "box"
+        half = np.full(3, g.uniform(rmin, rmax)) if kind == "sphere" else g.uniform(rmin, rmax, size=3)
+        center = g.uniform(lo + half, hi - half)
+        prims.append({"kind": kind, "center": center, "half": half, "colour": g.uniform(0.0, 1.0, size=3)})
+    return prims
+
+
+def voxelize_primitives(prims, X, alpha=1.0):
+    """(X,X,X,4) float32 grid of the scene at the grid's cell positions (world_axis): a cell
+    inside a primitive takes its colour (later primitives over earlier ones) and `alpha`;
+    cells outside every primitive are 0."""
+    pd = points_distance_for(X)
+    ax = world_axis(X, pd)
+    grid = np.zeros((X, X, X, 4), np.float32)
+    for p in prims:
+        c, h = p["center"], p["half"]
+        sl = []
+        for a in range(3):  # index range of the primitive's bounding box
+            idx = np.flatnonzero(np.abs(ax - c[a]) <= h[a])
+            sl.append(slice(int(idx[0]), int(idx[-1]) + 1) if idx.size else slice(0, 0))
+        if any(t.stop <= t.start for t in sl):
+            continue
+        dx, dy, dz = (ax[sl[a]] - c[a] for a in range(3))
+        if p["kind"] == "sphere":
+            inside = (dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2) <= h[0] ** 2
+        else:
+            inside = np.ones((dx.size, dy.size, dz.size), bool)
+        sub = grid[sl[0], sl[1], sl[2]]
+        sub[inside, :3] = p["colour"].astype(np.float32)
+        sub[inside, 3] = alpha
+    return torch.from_numpy(grid)
+
+
+def pixel_uv(H, W):
+    """(H*W, 2) float32 uniforms whose target pixel (src/ray_sampling.py:238-248: row =
+    round(W*v), col = round(H*u)) is (row, col), row-major: u = col/H, v = row/W."""
+    rows = torch.arange(H, dtype=torch.float32) / W
+    cols = torch.arange(W, dtype=torch.float32) / H
+    v = rows[:, None].expand(H, W)
+    u = cols[None, :].expand(H, W)
+    return torch.stack([u, v], -1).reshape(-1, 2).contiguous()
+
+
 @dataclass
 class Workload:
     name: str

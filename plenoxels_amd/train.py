@@ -49,13 +49,14 @@ class Trainer:
     storage_dtype bfloat16/float16 renders the full-resolution steps from a reduced-precision
     copy of the grid that Adam writes in the same pass (FP32 master and moments).
     uniforms_fn(step) -> (B,R,2) uniforms overrides torch.rand (tests feed fixed ones);
-    init_grid replaces the all-zero initial grid.
+    init_grid replaces the all-zero initial grid; near (first sample at near + delta) and
+    t_stop (early ray termination) are the march extensions, both 0 in the reference.
     """
 
     def __init__(self, imgs, transform_matricies, camera_angle_x, *, gridsize, points_distance,
                  number_of_rays, num_samples, delta_step, lr, tv, beta, even_spread=False, device="cuda",
                  progressive_growing=True, interp="nearest", storage_dtype=torch.float32, uniforms_fn=None,
-                 init_grid=None):
+                 init_grid=None, near=0.0, t_stop=0.0):
         self.dev = torch.device(device)
         self.imgs = imgs.to(self.dev).float().contiguous()
         self.c2w = transform_matricies.to(self.dev).float().contiguous()
@@ -70,6 +71,7 @@ class Trainer:
         self.progressive = bool(progressive_growing)
         self.interp = interp
         self.uniforms_fn = uniforms_fn
+        self.near, self.t_stop = float(near), float(t_stop)
         self.B = self.imgs.shape[0]
         X, Y, Z = self.dims
         self.grid = torch.zeros((X, Y, Z, 4), dtype=torch.float32, device=self.dev)
@@ -118,7 +120,7 @@ class Trainer:
             fill_(grad, 0.0)
         uv = self._uniforms(i).reshape(self.B * self.R, 2).contiguous()
         cam = CameraRays(self.cam_ids, uv, self.c2w, self.theta, self.imgs)
-        cfg = MarchConfig(self.S, self.delta, gmin, pd, self.interp)
+        cfg = MarchConfig(self.S, self.delta, gmin, pd, self.interp, self.near, self.t_stop)
         n_rays = self.B * self.R
         train_step(grid, grad, cfg=cfg, camera=cam, ray_sq_err=self.sq, beta=self.beta if self.beta > 0 else 0.0,
                    ray_beta=self.bsum if self.beta > 0 else None)
