@@ -10,9 +10,12 @@ the upper hemisphere of radius 4, 256 samples per ray from t = 2 + 4k/256, trili
 lookup, occupancy skipping, early termination at transmittance 1e-4.  Synthetic, seeded
 (grid 0, cameras 1, ray uniforms 2, targets 3).
 
-One step = zero-fill of the FP32 grid gradient + pxl_train_step (ray generation from
-(camera, u, v), target gather, sample placement, gather, compositing, MSE gradient,
-backward scatter) + the loss reduction.  Adam is not part of the step (reference
+One step = pxl_train_step_blocks (ray generation from (camera, u, v), target gather, sample
+placement, gather, compositing, MSE gradient, backward scatter into per-floor-cell corner
+blocks) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
+zero-fill -- and zeroes the other of two alternating block buffers) + the loss reduction.  --scatter direct times the
+zero-fill + pxl_train_step with atomics straight into the gradient instead (nearest always
+scatters directly: one cell per sample).  Adam is not part of the step (reference
 scripts/train.py:182 runs it after the timed chain).
 
 Baselines (outside the timed region of the fused path):
@@ -34,8 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from plenoxels_amd import (CameraRays, MarchConfig, _lib, camera_rays, fill_,  # noqa: E402
-                           occupancy_dilate, pack_occupancy, train_step)
+from plenoxels_amd import (CameraRays, CornerBlockPair, MarchConfig, _lib, blocks_reduce,  # noqa: E402
+                           camera_rays, fill_, occupancy_dilate, pack_occupancy, train_step)
 from plenoxels_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -54,6 +57,8 @@ def parse():
     ap.add_argument("--interp", default="trilinear", choices=["trilinear", "nearest"])
     ap.add_argument("--dtype", default="float32", choices=["float32", "bfloat16", "float16"])
     ap.add_argument("--rays", type=int, default=1 << 18)
+    ap.add_argument("--scatter", default="blocks", choices=["blocks", "direct"],
+                    help="trilinear gradient path: corner blocks + reduce, or atomics into the grid")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--plain-chunk", type=int, default=32768)
     ap.add_argument("--cpu-rays", type=int, default=2048)
@@ -89,20 +94,32 @@ class Setup:
         R = w.cam_ids.shape[0]
         self.R = R
         self.grad = torch.zeros(self.grid.shape, dtype=torch.float32, device=dev)
+        self.use_blocks = args.scatter == "blocks" and w.interp == "trilinear"
+        self.pair = CornerBlockPair(self.grid) if self.use_blocks else None
         self.out = torch.empty(R, 4, device=dev)
         self.sq = torch.empty(R, device=dev)
         torch.cuda.synchronize()
         log(f"setup {time.time() - t0:.1f}s: grid {tuple(self.grid.shape)} {self.grid.dtype}, "
             f"occupancy {w.occupancy.float().mean().item():.4f}, rays {R}, S {w.num_samples}, {w.interp}")
 
-    def step(self, k_start=None, k_end=None):
-        fill_(self.grad, 0.0)
-        if k_start is not None:
-            k_start.record()
-        train_step(self.grid, self.grad, cfg=self.cfg, camera=self.camera, occupancy=self.occ_bits,
-                   out=self.out, ray_sq_err=self.sq)
-        if k_end is not None:
-            k_end.record()
+    def step(self, ev=None):
+        """One step; ev = (train start, train end, reduce start, reduce end) events or None."""
+        if not self.use_blocks:
+            fill_(self.grad, 0.0)
+        if ev is not None:
+            ev[0].record()
+        train_step(self.grid, None if self.use_blocks else self.grad, cfg=self.cfg, camera=self.camera,
+                   occupancy=self.occ_bits, out=self.out, ray_sq_err=self.sq,
+                   blocks=self.pair.blocks if self.use_blocks else None)
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+        if self.use_blocks:
+            blocks_reduce(self.grid, self.pair.blocks, self.grad, occupancy=self.occ_bits, accumulate=False,
+                          clear=self.pair.other)
+            self.pair.flip()
+        if ev is not None:
+            ev[3].record()
         return self.sq.sum() / (4 * self.R)
 
 
@@ -110,15 +127,16 @@ def timed_fused(S, steps, warmup):
     for _ in range(warmup):
         S.step()
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        S.step(*ev[i])
+        S.step(ev[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
-    return wall, float(np.mean(kern_ms)), float(np.min(kern_ms))
+    kern_ms = [e[0].elapsed_time(e[1]) for e in ev]
+    red_ms = [e[2].elapsed_time(e[3]) for e in ev]
+    return wall, float(np.mean(kern_ms)), float(np.min(kern_ms)), float(np.mean(red_ms))
 
 
 def algorithmic_bytes(S, args):
@@ -231,7 +249,8 @@ def cpu_baseline(S, args):
 
 def load_traffic(args):
     """HBM bytes per launch of the train-step kernel from the committed PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"traffic_sparse256_{args.interp}_{args.dtype}.json")
+    scatter = args.scatter if args.interp == "trilinear" else "direct"
+    path = os.path.join(ROOT, "profiles", f"traffic_sparse256_{args.interp}_{args.dtype}_{scatter}.json")
     if not os.path.exists(path) or args.rays != (1 << 18):
         return None
     with open(path) as f:
@@ -259,10 +278,11 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
         return
     S = Setup(args, dev)
-    wall, kern_ms, kern_min_ms = timed_fused(S, args.steps, args.warmup)
+    wall, kern_ms, kern_min_ms, red_ms = timed_fused(S, args.steps, args.warmup)
     ms_per_step = wall * 1e3 / args.steps
     value = S.R / (wall / args.steps)
-    log(f"fused: {ms_per_step:.3f} ms/step, train-step kernel {kern_ms:.3f} ms (min {kern_min_ms:.3f})")
+    log(f"fused: {ms_per_step:.3f} ms/step, train-step kernel {kern_ms:.3f} ms (min {kern_min_ms:.3f}), "
+        f"blocks reduce {red_ms:.3f} ms")
 
     roofline = None
     extra = {}
@@ -275,6 +295,8 @@ def main():
                     "algorithmic_bytes": b_alg}
         extra.update({"mean_processed_samples_per_ray": round(mean_proc, 2),
                       "mean_distinct_cells_per_ray": round(mean_cells, 2)})
+    if S.use_blocks:
+        extra["blocks_reduce_ms"] = round(red_ms, 4)
     plain = None
     cpu = None
     if not args.no_baselines:
@@ -293,7 +315,8 @@ def main():
         "data": "synthetic (seeded blob field, value-noise occupancy, random targets)",
         "config": {"workload": "sparse256", "grid": [256, 256, 256], "rays": S.R, "samples_per_ray": 256,
                    "interp": args.interp, "occupancy": "~5%", "t_stop": 1e-4, "cameras": 100,
-                   "image": [800, 800]},
+                   "image": [800, 800],
+                   "scatter": "corner blocks + reduce" if S.use_blocks else "direct atomics + zero-fill"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_gpu": plain,

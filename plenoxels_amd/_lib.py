@@ -65,6 +65,10 @@ SIGNATURES = {
     "pxl_render_bwd": (ctypes.c_int, [_GP, _MP, _RP, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pxl_train_step": (ctypes.c_int, [_GP, _MP, _RP, ctypes.POINTER(Loss), c_void_p, c_void_p,
                                       c_void_p]),
+    "pxl_train_step_blocks": (ctypes.c_int, [_GP, _MP, _RP, ctypes.POINTER(Loss), c_void_p, c_void_p,
+                                             c_void_p]),
+    "pxl_blocks_floats": (ctypes.c_int64, [_GP]),
+    "pxl_blocks_reduce": (ctypes.c_int, [_GP, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, c_void_p, c_void_p]),

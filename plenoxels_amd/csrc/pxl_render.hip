@@ -16,6 +16,7 @@
 // in LDS and flushed four lanes per cell.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -417,6 +418,118 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
   out[384 + lane] = wave_shift_down(x, -1.f);
 }
 
+// Corner blocks -> gradient.  Block F (8 float4) holds, for corner j = 4bx + 2by + bz of
+// floor cell F (b = 0: the ceil index, F + 1 wrapped; b = 1: F itself), the four channel
+// gradients.  Output cell c therefore collects corner j from F = c - (1-bx, 1-by, 1-bz),
+// wrapped, summed in the fixed order j = 0..7; blocks whose floor cell is unoccupied were never
+// written and are skipped.
+//
+// A workgroup covers up to 256 cells of one (x, y) row, one thread per cell along z.  The four
+// source rows m = 2bx + by are (x-1+bx, y-1+by); their occupancy words for the workgroup's z
+// range are staged in LDS.  Each lane reads, from the block of its own z in source row m, the
+// adjacent corner pair j = 2m, 2m+1 (32 bytes, one memory segment): corner 2m+1 (bz = 1) is its
+// own, corner 2m (bz = 0) belongs to cell z+1 and moves one lane up by a DPP shift; the first
+// lane of each wave reads its z-1 corner itself.  Rows are numbered so that the workgroups in
+// flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the same blocks.
+// `clear` (a second block buffer, not read in this pass) has the blocks of the workgroup's
+// occupied cells zeroed in the same pass by coalesced stores, ready for the next training step.
+constexpr int kReduceCells = 256;
+constexpr int kReduceWords = 10;  // words spanning kReduceCells + 1 bits at any alignment
+
+__global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __restrict__ blocks,
+                                                                const uint32_t* __restrict__ occ,
+                                                                int X, int Y, int Z, int accumulate,
+                                                                float4* __restrict__ clear,
+                                                                float4* __restrict__ grad) {
+  __shared__ uint32_t s_w[4][kReduceWords];
+  const uint32_t nzc = ((uint32_t)Z + kReduceCells - 1) / kReduceCells;
+  const uint32_t n_units = (uint32_t)X * (uint32_t)Y * nzc;
+  const uint32_t n_wg = ((n_units + 7u) / 8u) * 8u, per_xcd = n_wg / 8u;
+  const int tid = (int)threadIdx.x, lane = tid & 63;
+  for (uint32_t wg = blockIdx.x; wg < n_wg; wg += gridDim.x) {
+    const uint32_t u = (wg % 8u) * per_xcd + wg / 8u;  // XCD-contiguous unit order
+    if (u >= n_units) continue;  // padding units of the last XCD share (uniform per workgroup)
+    const uint32_t zc = u % nzc, rxy = u / nzc;
+    const uint32_t x = rxy / (uint32_t)Y, y = rxy % (uint32_t)Y;
+    const uint32_t z0 = zc * kReduceCells;
+    const uint32_t nz = min((uint32_t)kReduceCells, (uint32_t)Z - z0);
+    const uint32_t xm = x == 0 ? (uint32_t)X - 1u : x - 1u, ym = y == 0 ? (uint32_t)Y - 1u : y - 1u;
+    const uint32_t B[4] = {(xm * (uint32_t)Y + ym) * (uint32_t)Z, (xm * (uint32_t)Y + y) * (uint32_t)Z,
+                           (x * (uint32_t)Y + ym) * (uint32_t)Z, (x * (uint32_t)Y + y) * (uint32_t)Z};
+    // occupancy words of cells B[m] + [z0 - 1 (or z0), z0 + nz) in LDS
+    const uint32_t first = z0 == 0 ? 0u : z0 - 1u;
+    if (occ && tid < 4 * kReduceWords) {
+      const int m = tid / kReduceWords, i = tid % kReduceWords;
+      const uint32_t w0 = (B[m] + first) >> 5, w1 = (B[m] + z0 + nz - 1u) >> 5;
+      if (w0 + (uint32_t)i <= w1) s_w[m][i] = occ[w0 + (uint32_t)i];
+    }
+    __syncthreads();
+    auto bit = [&](int m, uint32_t zz) -> bool {  // zz in [first, z0 + nz)
+      if (!occ) return true;
+      const uint32_t cell = B[m] + zz;
+      return (s_w[m][(cell >> 5) - ((B[m] + first) >> 5)] >> (cell & 31u)) & 1u;
+    };
+    const uint32_t z = z0 + (uint32_t)tid;
+    const bool active = (uint32_t)tid < nz;
+    // all block loads first (independent, in flight together), then the shifts and the sum
+    float4 lo[4], hi[4], halo[4];  // corners 2m (bz = 0), 2m+1 (bz = 1); lane 0: corner 2m at z-1
+    const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      lo[m] = hi[m] = halo[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (active && bit(m, z)) {
+        const float4* b = blocks + (size_t)(B[m] + z) * 8u + 2 * m;
+        lo[m] = b[0];
+        hi[m] = b[1];
+      }
+      if (lane == 0 && active && (z == 0 ? occupied(occ, B[m] + zp) : bit(m, zp)))
+        halo[m] = blocks[(size_t)(B[m] + zp) * 8u + 2 * m];
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float4 up;  // corner 2m of the block at z-1: lane l-1's lo, or lane 0's own halo load
+      up.x = wave_shift_up(lo[m].x, halo[m].x);
+      up.y = wave_shift_up(lo[m].y, halo[m].y);
+      up.z = wave_shift_up(lo[m].z, halo[m].z);
+      up.w = wave_shift_up(lo[m].w, halo[m].w);
+      acc.x += up.x;
+      acc.y += up.y;
+      acc.z += up.z;
+      acc.w += up.w;
+      acc.x += hi[m].x;
+      acc.y += hi[m].y;
+      acc.z += hi[m].z;
+      acc.w += hi[m].w;
+    }
+    if (active) {
+      const uint32_t c = B[3] + z;
+      if (accumulate) {
+        const float4 g = grad[c];
+        acc = make_float4(g.x + acc.x, g.y + acc.y, g.z + acc.z, g.w + acc.w);
+      }
+      grad[c] = acc;
+    }
+    if (clear) {
+      float4* cb = clear + (size_t)(B[3] + z0) * 8u;
+      for (uint32_t q = (uint32_t)tid; q < nz * 8u; q += kReduceCells)
+        if (bit(3, z0 + (q >> 3))) cb[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();  // s_w is rewritten by the next unit
+  }
+}
+
+// Zero every block of an occupied floor cell (every block without occupancy).  Eight lanes per
+// 128-byte block, so each store instruction covers eight whole consecutive blocks.
+__global__ __launch_bounds__(256) void k_blocks_zero(float4* __restrict__ blocks,
+                                                     const uint32_t* __restrict__ occ,
+                                                     int64_t n_cells) {
+  const int64_t n = n_cells * 8;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    if (occupied(occ, (uint32_t)(t >> 3))) blocks[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 }  // namespace pxl
 
 // ============================================================================= host ABI
@@ -476,6 +589,10 @@ using pxl_host::check_grid;
 using pxl_host::fail;
 using pxl_host::hip_fail;
 using pxl_host::kMaxCells;
+using pxl_host::kMaxBlockCells;
+using pxl_host::check_dims;
+using pxl_host::launched;
+using pxl_host::stride_blocks;
 
 constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // 2^24 blocks of 4 rays
 
@@ -642,14 +759,20 @@ int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
   return rc;
 }
 
-int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
-                   const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream) {
+static int train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                      const pxl_loss* loss, float* out_rgba, float* grad_grid, bool blocks,
+                      void* stream) {
   int rc;
   if (!loss) return fail(PXL_E_INVALID, "loss descriptor is NULL");
   if ((rc = check_grid(grid)) || (rc = check_march(march)) ||
       (rc = check_rays(rays, loss->targets == nullptr)))
     return rc;
-  if (!grad_grid) return fail(PXL_E_INVALID, "grad_grid is NULL");
+  if (!grad_grid) return fail(PXL_E_INVALID, blocks ? "blocks is NULL" : "grad_grid is NULL");
+  if (blocks && march->interp != PXL_INTERP_TRILINEAR)
+    return fail(PXL_E_INVALID, "corner blocks hold trilinear gradients (interp must be trilinear)");
+  if (blocks && grid->dims[0] * grid->dims[1] * grid->dims[2] > kMaxBlockCells)
+    return fail(PXL_E_SIZE, "corner blocks index floor_cell*8 + corner in 32 bits: at most %lld cells",
+                (long long)kMaxBlockCells);
   if (!std::isfinite(loss->grad_scale) || !std::isfinite(loss->beta_scale))
     return fail(PXL_E_INVALID, "grad_scale and beta_scale must be finite");
   if (march->n_samples > 512)
@@ -657,6 +780,7 @@ int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
                 "pxl_render_fwd + pxl_render_bwd", march->n_samples);
   if (rays->n_rays == 0) return PXL_OK;
   pxl::KParams P = make_params(grid, march, rays);
+  P.corner_blocks = blocks ? 1 : 0;
   pxl::LossParams LP{loss->targets, loss->grad_scale, loss->beta_scale, loss->ray_sq_err,
                      loss->ray_beta};
   hipStream_t s = (hipStream_t)stream;
@@ -676,9 +800,50 @@ int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
       else
         hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 8>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
                            LQ, out_rgba, grad_grid);
-    }, "pxl_train_step");
+    }, blocks ? "pxl_train_step_blocks" : "pxl_train_step");
   });
   return rc;
+}
+
+int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                   const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream) {
+  return train_step(grid, march, rays, loss, out_rgba, grad_grid, false, stream);
+}
+
+int pxl_train_step_blocks(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
+                          const pxl_loss* loss, float* out_rgba, float* blocks, void* stream) {
+  return train_step(grid, march, rays, loss, out_rgba, blocks, true, stream);
+}
+
+int64_t pxl_blocks_floats(const pxl_grid* grid) {
+  if (!grid || grid->dims[0] <= 0 || grid->dims[1] <= 0 || grid->dims[2] <= 0) return -1;
+  return grid->dims[0] * grid->dims[1] * grid->dims[2] * 32;
+}
+
+int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int32_t accumulate,
+                      float* clear, void* stream) {
+  int rc;
+  if (!grid) return fail(PXL_E_INVALID, "grid descriptor is NULL");
+  if ((rc = check_dims(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  if (!blocks || !grad_grid) return fail(PXL_E_INVALID, "blocks and grad_grid are required");
+  const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
+  if (n > kMaxBlockCells)
+    return fail(PXL_E_SIZE, "corner blocks index floor_cell*8 + corner in 32 bits: at most %lld cells",
+                (long long)kMaxBlockCells);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t units = grid->dims[0] * grid->dims[1] * ((grid->dims[2] + pxl::kReduceCells - 1) / pxl::kReduceCells);
+  const unsigned n_wg = (unsigned)std::min<int64_t>(((units + 7) / 8) * 8, int64_t(1) << 24);
+  const bool separate = clear == blocks;  // the pass reads `blocks`: clear them afterwards
+  hipLaunchKernelGGL(pxl::k_blocks_reduce, dim3(n_wg), dim3(pxl::kReduceCells), 0, s,
+                     reinterpret_cast<const float4*>(blocks), grid->occupancy, (int)grid->dims[0],
+                     (int)grid->dims[1], (int)grid->dims[2], accumulate ? 1 : 0,
+                     separate ? nullptr : reinterpret_cast<float4*>(clear),
+                     reinterpret_cast<float4*>(grad_grid));
+  if ((rc = launched("pxl_blocks_reduce"))) return rc;
+  if (!separate) return PXL_OK;
+  hipLaunchKernelGGL(pxl::k_blocks_zero, dim3(stride_blocks(n * 8, 256)), dim3(256), 0, s,
+                     reinterpret_cast<float4*>(blocks), grid->occupancy, n);
+  return launched("pxl_blocks_reduce");
 }
 
 int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,

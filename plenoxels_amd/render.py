@@ -205,9 +205,61 @@ def camera_rays(camera, want_targets=True):
     return o, d, t
 
 
+def corner_blocks(grid):
+    """Zeroed corner blocks for `grid` ((X*Y*Z, 32) float32 on its device): the accumulation
+    buffer of train_step(..., blocks=...), kept zero between steps by blocks_reduce."""
+    X, Y, Z = (int(v) for v in grid.shape[:3])
+    return torch.zeros((X * Y * Z, 32), dtype=torch.float32, device=grid.device)
+
+
+class CornerBlockPair:
+    """Two corner-block buffers used alternately: each reduce sums the active buffer and zeroes
+    the other one in the same pass, so no separate clearing pass is needed."""
+
+    def __init__(self, grid):
+        self.buffers = [corner_blocks(grid), corner_blocks(grid)]
+        self.active = 0
+
+    @property
+    def blocks(self):
+        return self.buffers[self.active]
+
+    @property
+    def other(self):
+        return self.buffers[1 - self.active]
+
+    def flip(self):
+        self.active = 1 - self.active
+
+
+def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clear=None):
+    """grad_grid (+)= the gradient held in corner blocks (pxl_blocks_reduce).  clear: the
+    blocks to zero for the next step -- default `blocks` itself (a second pass); another
+    buffer is zeroed within the same pass.  occupancy: the one the training steps used."""
+    L = _lib.lib()
+    _check_blocks(grid, blocks, grad_grid)
+    if clear is None:
+        clear = blocks
+    elif clear is not blocks:
+        _check_blocks(grid, clear, None)
+    g = _grid_struct(grid, occupancy)
+    _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid), 1 if accumulate else 0,
+                                   _lib.ptr(clear), _lib.stream_handle(grid.device)))
+
+
+def _check_blocks(grid, blocks, grad_grid):
+    X, Y, Z = (int(v) for v in grid.shape[:3])
+    if (blocks.dtype != torch.float32 or not blocks.is_contiguous() or blocks.numel() != X * Y * Z * 32
+            or blocks.device != grid.device):
+        raise ValueError("blocks must be contiguous float32 with X*Y*Z*32 elements on the grid's device")
+    if grad_grid is not None and (grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != (X, Y, Z, 4)
+                                  or not grad_grid.is_contiguous()):
+        raise ValueError("grad_grid must be a contiguous (X,Y,Z,4) float32 tensor")
+
+
 def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, targets=None,
                n_total_rays=None, occupancy=None, out=None, ray_sq_err=None, beta=0.0,
-               n_total_samples=None, ray_beta=None):
+               n_total_samples=None, ray_beta=None, blocks=None, overwrite=False, clear=None):
     """Render + loss gradient + backward in one kernel pass (scripts/train.py:130-181 minus
     the optimiser and the TV term).
 
@@ -217,9 +269,20 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
     None to gather them from camera.imgs.  n_total_rays / n_total_samples: the normalisers
     of the full batch this call is part of (default: this call's).  ray_sq_err (R,) and
     ray_beta (R,) receive per-ray loss partials when given (see pxl_loss in pxl.h).
+
+    blocks (trilinear only): zeroed corner blocks (corner_blocks(grid)); the gradient is
+    accumulated there and reduced into grad_grid before returning -- grad_grid is then
+    overwritten when overwrite=True (no zero-fill needed), else added to; `clear` as in
+    blocks_reduce.  grad_grid=None leaves it in the blocks for a later blocks_reduce
+    (several calls, one reduce).
     """
     L = _lib.lib()
-    if grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != tuple(grid.shape) or not grad_grid.is_contiguous():
+    if blocks is not None:
+        _check_blocks(grid, blocks, grad_grid)
+        if cfg.interp != "trilinear":
+            raise ValueError("corner blocks hold trilinear gradients")
+    elif grad_grid is None or grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != tuple(grid.shape) \
+            or not grad_grid.is_contiguous():
         raise ValueError("grad_grid must be a contiguous float32 tensor shaped like grid")
     if camera is not None:
         camera.validate()
@@ -246,8 +309,20 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
     loss.ray_beta = None if ray_beta is None else ray_beta.data_ptr()
     g = _grid_struct(grid, occupancy)
     m = cfg.to_struct()
-    _lib.check(L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
-                                _lib.ptr(out), _lib.ptr(grad_grid), _lib.stream_handle(grid.device)))
+    stream = _lib.stream_handle(grid.device)
+    if blocks is None:
+        _lib.check(L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
+                                    _lib.ptr(out), _lib.ptr(grad_grid), stream))
+    else:
+        _lib.check(L.pxl_train_step_blocks(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
+                                           _lib.ptr(out), _lib.ptr(blocks), stream))
+        if grad_grid is not None:
+            if clear is None:
+                clear = blocks
+            elif clear is not blocks:
+                _check_blocks(grid, clear, None)
+            _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid),
+                                           0 if overwrite else 1, _lib.ptr(clear), stream))
     return ray_sq_err
 
 

@@ -22,7 +22,7 @@ from . import _lib
 from .checkpoint import save_grid
 from .grid_functions import average_pool3d_grid  # noqa: F401  (reference-compatible API)
 from .optim import adam_step_, tv_backward_, tv_sumsq
-from .render import CameraRays, MarchConfig, beta_term_zero, fill_, train_step
+from .render import CameraRays, CornerBlockPair, MarchConfig, beta_term_zero, fill_, train_step
 
 RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
 STEPS_PER_FIELD = 5                                              # scripts/train.py:92
@@ -51,12 +51,15 @@ class Trainer:
     uniforms_fn(step) -> (B,R,2) uniforms overrides torch.rand (tests feed fixed ones);
     init_grid replaces the all-zero initial grid; near (first sample at near + delta) and
     t_stop (early ray termination) are the march extensions, both 0 in the reference.
+    corner_blocks (trilinear only) routes the gradient through per-floor-cell corner blocks
+    and their reduce (pxl_train_step_blocks), which replaces the zero-fill; it costs 128 B per
+    cell of device memory and a dense reduce pass per step.
     """
 
     def __init__(self, imgs, transform_matricies, camera_angle_x, *, gridsize, points_distance,
                  number_of_rays, num_samples, delta_step, lr, tv, beta, even_spread=False, device="cuda",
                  progressive_growing=True, interp="nearest", storage_dtype=torch.float32, uniforms_fn=None,
-                 init_grid=None, near=0.0, t_stop=0.0):
+                 init_grid=None, near=0.0, t_stop=0.0, corner_blocks=False):
         self.dev = torch.device(device)
         self.imgs = imgs.to(self.dev).float().contiguous()
         self.c2w = transform_matricies.to(self.dev).float().contiguous()
@@ -72,6 +75,10 @@ class Trainer:
         self.interp = interp
         self.uniforms_fn = uniforms_fn
         self.near, self.t_stop = float(near), float(t_stop)
+        if corner_blocks and interp != "trilinear":
+            raise ValueError("corner_blocks needs interp='trilinear'")
+        self.use_blocks = bool(corner_blocks)
+        self._blocks = {}  # grid shape -> CornerBlockPair (each reduce zeroes the other buffer)
         self.B = self.imgs.shape[0]
         X, Y, Z = self.dims
         self.grid = torch.zeros((X, Y, Z, 4), dtype=torch.float32, device=self.dev)
@@ -111,19 +118,29 @@ class Trainer:
             gmin = tuple(grid_coordinate(start, s, self.pd) for s in self.dims)
             pd = self.pd * stride
             grid = average_pool3d_grid(self.grid, receptive_field_size=k, stride=stride).detach()
-            grad = torch.zeros_like(grid)
+            grad = torch.empty_like(grid) if self.use_blocks else torch.zeros_like(grid)
         else:
             gmin = tuple(grid_coordinate(0, s, self.pd) for s in self.dims)
             pd = self.pd
             grid = self.grid if self.render_copy is None else self.render_copy
             grad = self.grad
-            fill_(grad, 0.0)
+            if not self.use_blocks:
+                fill_(grad, 0.0)
         uv = self._uniforms(i).reshape(self.B * self.R, 2).contiguous()
         cam = CameraRays(self.cam_ids, uv, self.c2w, self.theta, self.imgs)
         cfg = MarchConfig(self.S, self.delta, gmin, pd, self.interp, self.near, self.t_stop)
         n_rays = self.B * self.R
+        pair = None
+        if self.use_blocks:
+            key = tuple(grid.shape[:3])
+            if key not in self._blocks:
+                self._blocks[key] = CornerBlockPair(grid)
+            pair = self._blocks[key]
         train_step(grid, grad, cfg=cfg, camera=cam, ray_sq_err=self.sq, beta=self.beta if self.beta > 0 else 0.0,
-                   ray_beta=self.bsum if self.beta > 0 else None)
+                   ray_beta=self.bsum if self.beta > 0 else None, blocks=pair.blocks if pair else None,
+                   overwrite=True, clear=pair.other if pair else None)
+        if pair:
+            pair.flip()
         closs = self.sq.sum() / (4 * n_rays)
         out = {"closs": closs}
         loss = closs
