@@ -424,31 +424,34 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
 // wrapped, summed in the fixed order j = 0..7; blocks whose floor cell is unoccupied were never
 // written and are skipped.
 //
-// A workgroup covers up to 256 cells of one (x, y) row, one thread per cell along z.  The four
-// source rows m = 2bx + by are (x-1+bx, y-1+by); their occupancy words for the workgroup's z
-// range are staged in LDS.  Each lane reads, from the block of its own z in source row m, the
+// A workgroup covers up to 256 cells of one (x, y) row, one thread per cell along z; its four
+// waves work independently (no LDS, no barrier).  The four source rows m = 2bx + by are
+// (x-1+bx, y-1+by).  Each lane tests its own z in each source row (the z-1 bits come from the
+// lane below by a DPP shift) and reads, from the block of its own z in source row m, the
 // adjacent corner pair j = 2m, 2m+1 (32 bytes, one memory segment): corner 2m+1 (bz = 1) is its
 // own, corner 2m (bz = 0) belongs to cell z+1 and moves one lane up by a DPP shift; the first
-// lane of each wave reads its z-1 corner itself.  Rows are numbered so that the workgroups in
-// flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the same blocks.
-// `clear` (a second block buffer, not read in this pass) has the blocks of the workgroup's
-// occupied cells zeroed in the same pass by coalesced stores, ready for the next training step.
+// lane of each wave reads its z-1 bits and corners itself.  Rows are numbered so that the
+// workgroups in flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the
+// same blocks.  `clear` (a second block buffer, not read in this pass) has the blocks of each
+// wave's occupied cells zeroed in the same pass by coalesced stores, ready for the next step.
 constexpr int kReduceCells = 256;
-constexpr int kReduceWords = 10;  // words spanning kReduceCells + 1 bits at any alignment
+
+__device__ __forceinline__ uint32_t shift_up_u32(uint32_t v, uint32_t fill) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, DPP_WAVE_SHR1, 0xf, 0xf, false);
+}
 
 __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __restrict__ blocks,
                                                                 const uint32_t* __restrict__ occ,
                                                                 int X, int Y, int Z, int accumulate,
                                                                 float4* __restrict__ clear,
                                                                 float4* __restrict__ grad) {
-  __shared__ uint32_t s_w[4][kReduceWords];
   const uint32_t nzc = ((uint32_t)Z + kReduceCells - 1) / kReduceCells;
   const uint32_t n_units = (uint32_t)X * (uint32_t)Y * nzc;
   const uint32_t n_wg = ((n_units + 7u) / 8u) * 8u, per_xcd = n_wg / 8u;
   const int tid = (int)threadIdx.x, lane = tid & 63;
   for (uint32_t wg = blockIdx.x; wg < n_wg; wg += gridDim.x) {
     const uint32_t u = (wg % 8u) * per_xcd + wg / 8u;  // XCD-contiguous unit order
-    if (u >= n_units) continue;  // padding units of the last XCD share (uniform per workgroup)
+    if (u >= n_units) continue;
     const uint32_t zc = u % nzc, rxy = u / nzc;
     const uint32_t x = rxy / (uint32_t)Y, y = rxy % (uint32_t)Y;
     const uint32_t z0 = zc * kReduceCells;
@@ -456,34 +459,28 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
     const uint32_t xm = x == 0 ? (uint32_t)X - 1u : x - 1u, ym = y == 0 ? (uint32_t)Y - 1u : y - 1u;
     const uint32_t B[4] = {(xm * (uint32_t)Y + ym) * (uint32_t)Z, (xm * (uint32_t)Y + y) * (uint32_t)Z,
                            (x * (uint32_t)Y + ym) * (uint32_t)Z, (x * (uint32_t)Y + y) * (uint32_t)Z};
-    // occupancy words of cells B[m] + [z0 - 1 (or z0), z0 + nz) in LDS
-    const uint32_t first = z0 == 0 ? 0u : z0 - 1u;
-    if (occ && tid < 4 * kReduceWords) {
-      const int m = tid / kReduceWords, i = tid % kReduceWords;
-      const uint32_t w0 = (B[m] + first) >> 5, w1 = (B[m] + z0 + nz - 1u) >> 5;
-      if (w0 + (uint32_t)i <= w1) s_w[m][i] = occ[w0 + (uint32_t)i];
-    }
-    __syncthreads();
-    auto bit = [&](int m, uint32_t zz) -> bool {  // zz in [first, z0 + nz)
-      if (!occ) return true;
-      const uint32_t cell = B[m] + zz;
-      return (s_w[m][(cell >> 5) - ((B[m] + first) >> 5)] >> (cell & 31u)) & 1u;
-    };
     const uint32_t z = z0 + (uint32_t)tid;
     const bool active = (uint32_t)tid < nz;
+    const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
+    // occupancy bits: own[m] for (m, z); the lane below supplies (m, z-1), lane 0 loads it
+    uint32_t own = 0u, own_p = 0u;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (active && occupied(occ, B[m] + z)) own |= 1u << m;
+      if (lane == 0 && active && occupied(occ, B[m] + zp)) own_p |= 1u << m;
+    }
+    const uint32_t prev = shift_up_u32(own, own_p);  // bits of (m, z-1)
     // all block loads first (independent, in flight together), then the shifts and the sum
     float4 lo[4], hi[4], halo[4];  // corners 2m (bz = 0), 2m+1 (bz = 1); lane 0: corner 2m at z-1
-    const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       lo[m] = hi[m] = halo[m] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (active && bit(m, z)) {
+      if ((own >> m) & 1u) {
         const float4* b = blocks + (size_t)(B[m] + z) * 8u + 2 * m;
         lo[m] = b[0];
         hi[m] = b[1];
       }
-      if (lane == 0 && active && (z == 0 ? occupied(occ, B[m] + zp) : bit(m, zp)))
-        halo[m] = blocks[(size_t)(B[m] + zp) * 8u + 2 * m];
+      if (lane == 0 && ((prev >> m) & 1u)) halo[m] = blocks[(size_t)(B[m] + zp) * 8u + 2 * m];
     }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -510,12 +507,16 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
       }
       grad[c] = acc;
     }
-    if (clear) {
-      float4* cb = clear + (size_t)(B[3] + z0) * 8u;
-      for (uint32_t q = (uint32_t)tid; q < nz * 8u; q += kReduceCells)
-        if (bit(3, z0 + (q >> 3))) cb[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (clear) {  // this wave's 64 cells = 512 float4: lane l of pass k clears float4 64k + l
+      const uint32_t wz0 = z0 + (uint32_t)(tid & ~63);
+      float4* cb = clear + (size_t)(B[3] + wz0) * 8u;
+      const uint32_t own3 = (own >> 3) & 1u;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int cell = 8 * k + (lane >> 3);  // lane within the wave whose cell this is
+        if (__shfl(own3, cell)) cb[64 * k + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
-    __syncthreads();  // s_w is rewritten by the next unit
   }
 }
 
