@@ -10,9 +10,9 @@ the upper hemisphere of radius 4, 256 samples per ray from t = 2 + 4k/256, trili
 lookup, occupancy skipping, early termination at transmittance 1e-4.  Synthetic, seeded
 (grid 0, cameras 1, ray uniforms 2, targets 3).
 
-One step = pxl_train_step_blocks (ray generation from (camera, u, v), target gather, sample
-placement, gather, compositing, MSE gradient, backward scatter into per-floor-cell corner
-blocks) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
+One step = pxl_camera_rays (ray generation from (camera, u, v) and target gather) +
+pxl_train_step_blocks (sample placement, gather, compositing, MSE gradient, backward scatter
+into per-floor-cell corner blocks) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
 zero-fill -- and zeroes the other of two alternating block buffers) + the loss reduction.  --scatter direct times the
 zero-fill + pxl_train_step with atomics straight into the gradient instead (nearest always
 scatters directly: one cell per sample).  Adam is not part of the step (reference
@@ -106,10 +106,11 @@ class Setup:
         """One step; ev = (train start, train end, reduce start, reduce end) events or None."""
         if not self.use_blocks:
             fill_(self.grad, 0.0)
+        o, d, tgt = camera_rays(self.camera)
         if ev is not None:
             ev[0].record()
-        train_step(self.grid, None if self.use_blocks else self.grad, cfg=self.cfg, camera=self.camera,
-                   occupancy=self.occ_bits, out=self.out, ray_sq_err=self.sq,
+        train_step(self.grid, None if self.use_blocks else self.grad, cfg=self.cfg, origins=o, dirs=d,
+                   targets=tgt, occupancy=self.occ_bits, out=self.out, ray_sq_err=self.sq,
                    blocks=self.pair.blocks if self.use_blocks else None)
         if ev is not None:
             ev[1].record()
@@ -140,9 +141,9 @@ def timed_fused(S, steps, warmup):
 
 
 def algorithmic_bytes(S, args):
-    """Per-launch algorithmic bytes of pxl_train_step: per ray 48 B (cam id 4 + uv 8 in,
-    target 16, RGBA 16 out, squared error 4) plus K_r cells x 4 channels x (s_g read + 8 for
-    the FP32 gradient read-modify-write).  K_r counted exactly on the host from the seeded
+    """Per-launch algorithmic bytes of pxl_train_step: per ray 60 B (origin + direction 24 and
+    target 16 in, RGBA 16 and squared error 4 out) plus K_r cells x 4 channels x (s_g read + 8
+    for the FP32 gradient read-modify-write).  K_r counted exactly on the host from the seeded
     batch (pxl_count_cells_host) using the termination counts of the forward kernel."""
     w = S.w
     L = _lib.lib()
@@ -171,7 +172,7 @@ def algorithmic_bytes(S, args):
                                       S.R, nproc.ctypes.data, counts.ctypes.data, nt))
     s_g = S.grid.element_size()
     per_cell = 4 * (s_g + 8)
-    total = S.R * 48 + int(counts.sum()) * per_cell
+    total = S.R * 60 + int(counts.sum()) * per_cell
     log(f"accounting {time.time() - t0:.1f}s: mean processed samples/ray {nproc.mean():.1f}, "
         f"mean distinct cells/ray {counts.mean():.1f}")
     return total, float(nproc.mean()), float(counts.mean())

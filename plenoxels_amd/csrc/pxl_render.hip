@@ -241,18 +241,8 @@ struct LossParams {
   float* beta;
 };
 
-// Occupancy target of the training step (waves per SIMD; register budget 512 / waves).  The
-// trilinear kernels with up to four groups run 5 waves (the compiler alone allocates for 4
-// with FP16 storage), measured 2-7% faster on sparse256; the others take what the compiler
-// allocates.
 template <int DT, int INTERP, int NG>
-struct TrainWaves {
-  static constexpr int value = (INTERP == PXL_INTERP_TRILINEAR && NG <= 4) ? 5 : 1;
-};
-
-template <int DT, int INTERP, int NG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TrainWaves<DT, INTERP, NG>::value)))
-void k_train_step(KParams P, LossParams LP,
+__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
                                                         float* __restrict__ out,
                                                         float* __restrict__ grad) {
   const float* targets = LP.targets;

@@ -259,7 +259,8 @@ def _check_blocks(grid, blocks, grad_grid):
 
 def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, targets=None,
                n_total_rays=None, occupancy=None, out=None, ray_sq_err=None, beta=0.0,
-               n_total_samples=None, ray_beta=None, blocks=None, overwrite=False, clear=None):
+               n_total_samples=None, ray_beta=None, blocks=None, overwrite=False, clear=None,
+               fused_camera=False):
     """Render + loss gradient + backward in one kernel pass (scripts/train.py:130-181 minus
     the optimiser and the TV term).
 
@@ -269,6 +270,11 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
     None to gather them from camera.imgs.  n_total_rays / n_total_samples: the normalisers
     of the full batch this call is part of (default: this call's).  ray_sq_err (R,) and
     ray_beta (R,) receive per-ray loss partials when given (see pxl_loss in pxl.h).
+
+    Camera-form rays are generated by pxl_camera_rays into (R,3)+(R,3)+(R,4) buffers first and
+    the step runs on explicit rays (the same bits; the separate generation pass takes the
+    dependent camera loads and the scattered target gather off the step's critical path, 16%
+    faster on sparse256); fused_camera=True generates them inside the step kernel instead.
 
     blocks (trilinear only): zeroed corner blocks (corner_blocks(grid)); the gradient is
     accumulated there and reduced into grad_grid before returning -- grad_grid is then
@@ -286,9 +292,14 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
         raise ValueError("grad_grid must be a contiguous float32 tensor shaped like grid")
     if camera is not None:
         camera.validate()
-        r = _rays_struct(camera=camera)
         if targets is None and camera.imgs is None:
             raise ValueError("targets or camera.imgs is required")
+        if fused_camera:
+            r = _rays_struct(camera=camera)
+        else:
+            origins, dirs, gathered = camera_rays(camera, want_targets=targets is None)
+            targets = gathered if targets is None else targets
+            r = _rays_struct(origins, dirs)
     else:
         r = _rays_struct(origins.contiguous(), dirs.contiguous())
     R = r.n_rays

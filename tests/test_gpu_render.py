@@ -235,13 +235,14 @@ def test_pack_occupancy_matches_oracle_bits():
     assert torch.equal(got, plain.occupancy_to_bits(occ))
 
 
+@pytest.mark.parametrize("fused_camera", [False, True])
 @pytest.mark.parametrize("t_stop", [0.0, 1e-4])
-def test_sparse256_subset_train_step_vs_plain(t_stop):
+def test_sparse256_subset_train_step_vs_plain(t_stop, fused_camera):
     """The benchmark workload itself (256^3, ~5% occupancy, trilinear, camera-form rays,
     targets gathered from resident images) on a 2048-ray subset, fused train step vs the
     plain chain on the CPU with the same rays."""
     from plenoxels_amd import occupancy_dilate, pack_occupancy, scenes
-    say("sparse256 subset t_stop", t_stop)
+    say("sparse256 subset t_stop", t_stop, "fused camera" if fused_camera else "separate ray pass")
     w = scenes.make_workload("sparse256", n_rays=1 << 18)
     n = 2048
     cam, uv = w.cam_ids[:n].contiguous(), w.uv[:n].contiguous()
@@ -262,7 +263,7 @@ def test_sparse256_subset_train_step_vs_plain(t_stop):
     cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance, "trilinear", w.near, t_stop)
     gg = torch.zeros_like(grid)
     out = torch.empty(n, 4, device=DEV)
-    train_step(grid, gg, cfg=cfg, camera=cr, occupancy=bits, out=out)
+    train_step(grid, gg, cfg=cfg, camera=cr, occupancy=bits, out=out, fused_camera=fused_camera)
     err = (out.cpu() - want.detach()).abs().max(dim=1).values
     say("  rgba: max err", err.max().item(), "rays > 1e-5:", int((err > 1e-5).sum()))
     ok, info = grad_close(gg.cpu(), g.grad)
