@@ -178,6 +178,14 @@ int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* ta
 int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z,
                          uint32_t* occ_out, void* stream);
 
+/* Occupancy from the grid's alpha channel: bit L of bits (ceil(XYZ/32) words, the layout of
+ * pxl_grid.occupancy) set when raw alpha >= 0 at cell L, or, with dilate, at any of its 8
+ * corners (the cells a trilinear sample with floor cell L reads).  Samples that this skips are
+ * exactly zero in the reference chain (clamp gives alpha 0 and a zero gradient mask), so
+ * passing it to the render kernels changes no finite result; NaN alpha counts as empty.
+ * grid->occupancy is not read. */
+int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, void* stream);
+
 /* Pack a (X,Y,Z) occupancy grid of bytes (nonzero = occupied) into the bit layout of
  * pxl_grid.occupancy: bit (L & 31) of word L >> 5.  out: ceil(n_cells/32) words. */
 int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream);
@@ -255,12 +263,16 @@ int pxl_grid_coords(int64_t X, int64_t Y, int64_t Z, float pd, float* coords, in
                     void* stream);
 
 /* src/grid_functions.py:173-181 (average_pool3d_grid): avg_pool3d with kernel k, stride s, no
- * padding over the (X,Y,Z,4) layout directly; out (Xo,Yo,Zo,4), Xo = (X-k)/s + 1. */
+ * padding over the (X,Y,Z,4) layout directly; out (Xo,Yo,Zo,4), Xo = (X-k)/s + 1.  Computed as
+ * the mean over k along z, y, then x (the window is separable); workspace: caller's scratch of
+ * pxl_avg_pool3d_workspace_floats(X,Y,Z,k,s) floats. */
+int64_t pxl_avg_pool3d_workspace_floats(int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s);
 int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s,
-                   float* out, void* stream);
-/* backward: grad_grid (X,Y,Z,4) += sum over windows containing a cell of grad_out / k^3 */
+                   float* out, float* workspace, void* stream);
+/* backward: grad_grid (X,Y,Z,4) += sum over windows containing a cell of grad_out / k^3 (the
+ * adjoint passes x, y, z); workspace as for the forward. */
 int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
-                       int32_t s, float* grad_grid, void* stream);
+                       int32_t s, float* grad_grid, float* workspace, void* stream);
 
 /* scripts/train.py:44-65 (tv_loss): sum_sq (1 double, device, accumulated) += sum over the
  * three axes of squared neighbour differences of grid (X,Y,Z,4). */

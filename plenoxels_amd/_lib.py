@@ -73,6 +73,7 @@ SIGNATURES = {
     "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, c_void_p, c_void_p]),
     "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_alpha_occupancy": (ctypes.c_int, [_GP, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_fill_f32": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p]),
     "pxl_sample_points": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64,
                                          ctypes.c_int32, ctypes.c_float, ctypes.c_float, c_void_p,
@@ -98,10 +99,12 @@ SIGNATURES = {
                                          c_void_p]),
     "pxl_grid_coords": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                        c_void_p, c_void_p, c_void_p]),
+    "pxl_avg_pool3d_workspace_floats": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                         ctypes.c_int32, ctypes.c_int32]),
     "pxl_avg_pool3d": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                      ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+                                      ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, c_void_p]),
     "pxl_avg_pool3d_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                          ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+                                          ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, c_void_p]),
     "pxl_tv_sumsq": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,
                                     c_void_p]),
     "pxl_tv_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,

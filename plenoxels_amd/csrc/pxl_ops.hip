@@ -366,58 +366,55 @@ __global__ void k_grid_coords(Dims d, float cx, float cy, float cz, float pd,
   }
 }
 
-// one block per output cell: the k^3 window summed by all threads, then reduced
-__global__ __launch_bounds__(256) void k_avg_pool(const float4* __restrict__ in, Dims d, Dims o,
-                                                   int k, int s, float4* __restrict__ out) {
-  __shared__ float4 red[256 / kWave];
-  const int64_t L = blockIdx.x;
-  const int64_t zo = L % o.Z, yo = (L / o.Z) % o.Y, xo = L / (o.Y * o.Z);
-  const int64_t kk = (int64_t)k * k * k;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int64_t w = threadIdx.x; w < kk; w += blockDim.x) {
-    const int64_t dz = w % k, dy = (w / k) % k, dx = w / ((int64_t)k * k);
-    const float4 v = in[cell_of(d, xo * s + dx, yo * s + dy, zo * s + dz)];
-    acc.x += v.x;
-    acc.y += v.y;
-    acc.z += v.z;
-    acc.w += v.w;
-  }
-  acc.x = wave_sum(acc.x);
-  acc.y = wave_sum(acc.y);
-  acc.z = wave_sum(acc.z);
-  acc.w = wave_sum(acc.w);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float4 t = red[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
-      t.x += red[i].x;
-      t.y += red[i].y;
-      t.z += red[i].z;
-      t.w += red[i].w;
+// A cubic window is separable: the pooled grid is the mean over k along z, then along y, then
+// along x, each pass a thread per output element of the array viewed as [outer][n][inner]
+// (consecutive threads take consecutive inner elements, so the y and x passes read and write
+// contiguously; the z pass has inner = 1 and its threads read overlapping runs of one row).
+// Each pass divides by k, so partial sums stay the size of the values.
+__global__ __launch_bounds__(256) void k_box_mean(const float4* __restrict__ in, int64_t outer,
+                                                  int64_t n, int64_t inner, int k, int s,
+                                                  int64_t n_out, float4* __restrict__ out) {
+  const float fk = (float)k;
+  PXL_GRID_STRIDE(L, outer * n_out * inner) {
+    const int64_t i = L % inner, o = (L / inner) % n_out, a = L / (inner * n_out);
+    const float4* src = in + (a * n + o * s) * inner + i;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int d = 0; d < k; ++d) {
+      const float4 v = src[(int64_t)d * inner];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
     }
-    const float q = (float)kk;
-    out[L] = make_float4(t.x / q, t.y / q, t.z / q, t.w / q);
+    out[L] = make_float4(acc.x / fk, acc.y / fk, acc.z / fk, acc.w / fk);
   }
 }
 
-// thread per output cell, for small windows
-__global__ void k_avg_pool_small(const float4* __restrict__ in, Dims d, Dims o, int k, int s,
-                                 float4* __restrict__ out) {
-  const float q = (float)((int64_t)k * k * k);
-  PXL_GRID_STRIDE(L, o.X * o.Y * o.Z) {
-    const int64_t zo = L % o.Z, yo = (L / o.Z) % o.Y, xo = L / (o.Y * o.Z);
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int dx = 0; dx < k; ++dx)
-      for (int dy = 0; dy < k; ++dy)
-        for (int dz = 0; dz < k; ++dz) {
-          const float4 v = in[cell_of(d, xo * s + dx, yo * s + dy, zo * s + dz)];
-          acc.x += v.x;
-          acc.y += v.y;
-          acc.z += v.z;
-          acc.w += v.w;
-        }
-    out[L] = make_float4(acc.x / q, acc.y / q, acc.z / q, acc.w / q);
+// The z pass (inner = 1): one workgroup per row loads the row into LDS with coalesced reads,
+// then each thread sums its windows from LDS (threads of the generic pass would each read a
+// separate memory segment, their windows being s cells apart).
+constexpr int kRowLdsCells = 2048;  // rows up to 2048 cells (32 KiB of LDS)
+
+__global__ __launch_bounds__(256) void k_box_mean_rows(const float4* __restrict__ in, int64_t rows,
+                                                       int n, int k, int s, int n_out,
+                                                       float4* __restrict__ out) {
+  __shared__ float4 s_row[kRowLdsCells];
+  const float fk = (float)k;
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_row[i] = in[r * n + i];
+    __syncthreads();
+    for (int o = threadIdx.x; o < n_out; o += blockDim.x) {
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int d = 0; d < k; ++d) {
+        const float4 v = s_row[o * s + d];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+      }
+      out[r * n_out + o] = make_float4(acc.x / fk, acc.y / fk, acc.z / fk, acc.w / fk);
+    }
+    __syncthreads();
   }
 }
 
@@ -429,31 +426,31 @@ __device__ __forceinline__ void window_range(int64_t x, int k, int s, int64_t n_
   if (hi > n_out - 1) hi = n_out - 1;
 }
 
-__global__ void k_avg_pool_bwd(const float4* __restrict__ gout, Dims d, Dims o, int k, int s,
-                               float4* __restrict__ grad) {
-  const float q = (float)((int64_t)k * k * k);
-  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
-    int64_t x0, x1, y0, y1, z0, z1;
-    window_range(x, k, s, o.X, x0, x1);
-    window_range(y, k, s, o.Y, y0, y1);
-    window_range(z, k, s, o.Z, z0, z1);
+// Adjoint of one k_box_mean pass: out[a][x][i] (+)= sum over the windows o containing x of
+// in[a][o][i] / k.
+__global__ __launch_bounds__(256) void k_box_mean_adjoint(const float4* __restrict__ in, int64_t outer,
+                                                          int64_t n, int64_t inner, int k, int s,
+                                                          int64_t n_out, int accumulate,
+                                                          float4* __restrict__ out) {
+  const float fk = (float)k;
+  PXL_GRID_STRIDE(L, outer * n * inner) {
+    const int64_t i = L % inner, x = (L / inner) % n, a = L / (inner * n);
+    int64_t lo, hi;
+    window_range(x, k, s, n_out, lo, hi);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int64_t a = x0; a <= x1; ++a)
-      for (int64_t b = y0; b <= y1; ++b)
-        for (int64_t c = z0; c <= z1; ++c) {
-          const float4 g = gout[cell_of(o, a, b, c)];
-          acc.x += g.x / q;
-          acc.y += g.y / q;
-          acc.z += g.z / q;
-          acc.w += g.w / q;
-        }
-    float4 cur = grad[L];
-    cur.x += acc.x;
-    cur.y += acc.y;
-    cur.z += acc.z;
-    cur.w += acc.w;
-    grad[L] = cur;
+    for (int64_t o = lo; o <= hi; ++o) {
+      const float4 g = in[(a * n_out + o) * inner + i];
+      acc.x += g.x;
+      acc.y += g.y;
+      acc.z += g.z;
+      acc.w += g.w;
+    }
+    acc = make_float4(acc.x / fk, acc.y / fk, acc.z / fk, acc.w / fk);
+    if (accumulate) {
+      const float4 c = out[L];
+      acc = make_float4(c.x + acc.x, c.y + acc.y, c.z + acc.z, c.w + acc.w);
+    }
+    out[L] = acc;
   }
 }
 
@@ -515,27 +512,70 @@ __device__ __forceinline__ uint16_t to_bf16_rne(float f) {
   return (uint16_t)(u >> 16);
 }
 
+// One element of torch.optim.Adam's update (see the host entry point for the scalar terms).
+__device__ __forceinline__ float adam_elem(float& p, float gi, float& m, float& v, float w1, float beta2,
+                                           float w2, float bc2_sqrt, float eps, float neg_step) {
+  // exp_avg.lerp_(grad, 1 - beta1): weight < 0.5 form  m + w * (g - m)
+  m = m + w1 * (gi - m);
+  v = (v * beta2) + (w2 * gi) * gi;  // addcmul: value * t1 * t2
+  const float denom = (__builtin_sqrtf(v) / bc2_sqrt) + eps;
+  p = p + neg_step * (m / denom);
+  return p;
+}
+
+__device__ __forceinline__ void store_half(void* half_out, int half_dtype, int64_t i, float x) {
+  if (half_dtype == PXL_DTYPE_BF16)
+    reinterpret_cast<uint16_t*>(half_out)[i] = to_bf16_rne(x);
+  else
+    reinterpret_cast<_Float16*>(half_out)[i] = (_Float16)x;
+}
+
+// Elements [0, 4*n4) four at a time with 16-byte loads and stores (every pointer 16-byte
+// aligned), then the tail [4*n4, n) one at a time.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, float* __restrict__ gabs, int64_t n, float w1,
+                       float* __restrict__ v, float* __restrict__ gabs, int64_t n, int64_t n4, float w1,
                        float beta2, float w2, float bc2_sqrt, float eps, float neg_step,
                        void* __restrict__ half_out, int half_dtype) {
-  PXL_GRID_STRIDE(i, n) {
+  PXL_GRID_STRIDE(i, n4) {
+    float4 P = reinterpret_cast<float4*>(p)[i];
+    const float4 G = reinterpret_cast<const float4*>(g)[i];
+    float4 M = reinterpret_cast<float4*>(m)[i];
+    float4 V = reinterpret_cast<float4*>(v)[i];
+    adam_elem(P.x, G.x, M.x, V.x, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.y, G.y, M.y, V.y, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.z, G.z, M.z, V.z, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.w, G.w, M.w, V.w, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    reinterpret_cast<float4*>(p)[i] = P;
+    reinterpret_cast<float4*>(m)[i] = M;
+    reinterpret_cast<float4*>(v)[i] = V;
+    if (gabs) {
+      float4 A = reinterpret_cast<float4*>(gabs)[i];
+      A = make_float4(A.x + fabsf(G.x), A.y + fabsf(G.y), A.z + fabsf(G.z), A.w + fabsf(G.w));
+      reinterpret_cast<float4*>(gabs)[i] = A;
+    }
+    if (half_out) {
+      if (half_dtype == PXL_DTYPE_BF16) {
+        const uint32_t lo = to_bf16_rne(P.x) | ((uint32_t)to_bf16_rne(P.y) << 16);
+        const uint32_t hi = to_bf16_rne(P.z) | ((uint32_t)to_bf16_rne(P.w) << 16);
+        reinterpret_cast<uint2*>(half_out)[i] = make_uint2(lo, hi);
+      } else {
+        store_half(half_out, half_dtype, 4 * i + 0, P.x);
+        store_half(half_out, half_dtype, 4 * i + 1, P.y);
+        store_half(half_out, half_dtype, 4 * i + 2, P.z);
+        store_half(half_out, half_dtype, 4 * i + 3, P.w);
+      }
+    }
+  }
+  PXL_GRID_STRIDE(j, n - 4 * n4) {
+    const int64_t i = 4 * n4 + j;
     const float gi = g[i];
-    // exp_avg.lerp_(grad, 1 - beta1): weight < 0.5 form  m + w * (g - m)
-    const float mi = m[i] + w1 * (gi - m[i]);
-    const float vi = (v[i] * beta2) + (w2 * gi) * gi;  // addcmul: value * t1 * t2
-    const float denom = (__builtin_sqrtf(vi) / bc2_sqrt) + eps;
-    const float pi = p[i] + neg_step * (mi / denom);
+    float pi = p[i], mi = m[i], vi = v[i];
+    adam_elem(pi, gi, mi, vi, w1, beta2, w2, bc2_sqrt, eps, neg_step);
     m[i] = mi;
     v[i] = vi;
     p[i] = pi;
     if (gabs) gabs[i] += fabsf(gi);
-    if (half_out) {
-      if (half_dtype == PXL_DTYPE_BF16)
-        reinterpret_cast<uint16_t*>(half_out)[i] = to_bf16_rne(pi);
-      else
-        reinterpret_cast<_Float16*>(half_out)[i] = (_Float16)pi;
-    }
+    if (half_out) store_half(half_out, half_dtype, i, pi);
   }
 }
 
@@ -758,33 +798,52 @@ int pxl_grid_coords(int64_t X, int64_t Y, int64_t Z, float pd, float* coords, in
   return launched("pxl_grid_coords");
 }
 
+int64_t pxl_avg_pool3d_workspace_floats(int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s) {
+  if (X <= 0 || Y <= 0 || Z <= 0 || k < 1 || s < 1 || k > X || k > Y || k > Z) return -1;
+  const int64_t oy = (Y - k) / s + 1, oz = (Z - k) / s + 1;
+  return (X * Y * oz + X * oy * oz) * 4;
+}
+
 int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s,
-                   float* out, void* stream) {
+                   float* out, float* workspace, void* stream) {
   int rc;
-  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grid && out, "grid and out required")) ||
+  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grid && out && workspace, "grid, out and workspace required")) ||
       (rc = need(k >= 1 && s >= 1 && k <= X && k <= Y && k <= Z, "need 1 <= k <= dims, s >= 1")))
     return rc;
-  const Dims o{(X - k) / s + 1, (Y - k) / s + 1, (Z - k) / s + 1};
-  const int64_t n_out = o.X * o.Y * o.Z;
-  if ((int64_t)k * k * k <= 512 || n_out > (int64_t(1) << 20))
-    hipLaunchKernelGGL(pxl::k_avg_pool_small, dim3(stride_blocks(n_out, 256)), dim3(256), 0,
-                       (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, o, k, s, (float4*)out);
+  const int64_t ox = (X - k) / s + 1, oy = (Y - k) / s + 1, oz = (Z - k) / s + 1;
+  float4* a = reinterpret_cast<float4*>(workspace);  // (X, Y, oz)
+  float4* b = a + X * Y * oz;                         // (X, oy, oz)
+  hipStream_t st = (hipStream_t)stream;
+  if (Z <= pxl::kRowLdsCells)
+    hipLaunchKernelGGL(pxl::k_box_mean_rows, dim3((unsigned)std::min<int64_t>(X * Y, int64_t(1) << 20)),
+                       dim3(256), 0, st, (const float4*)grid, X * Y, (int)Z, k, s, (int)oz, a);
   else
-    hipLaunchKernelGGL(pxl::k_avg_pool, dim3((unsigned)n_out), dim3(256), 0, (hipStream_t)stream,
-                       (const float4*)grid, Dims{X, Y, Z}, o, k, s, (float4*)out);
+    hipLaunchKernelGGL(pxl::k_box_mean, dim3(stride_blocks(X * Y * oz, 256)), dim3(256), 0, st,
+                       (const float4*)grid, X * Y, Z, (int64_t)1, k, s, oz, a);
+  hipLaunchKernelGGL(pxl::k_box_mean, dim3(stride_blocks(X * oy * oz, 256)), dim3(256), 0, st,
+                     (const float4*)a, X, Y, oz, k, s, oy, b);
+  hipLaunchKernelGGL(pxl::k_box_mean, dim3(stride_blocks(ox * oy * oz, 256)), dim3(256), 0, st,
+                     (const float4*)b, (int64_t)1, X, oy * oz, k, s, ox, (float4*)out);
   return launched("pxl_avg_pool3d");
 }
 
 int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
-                       int32_t s, float* grad_grid, void* stream) {
+                       int32_t s, float* grad_grid, float* workspace, void* stream) {
   int rc;
-  if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grad_out && grad_grid, "pointers required")) ||
+  if ((rc = pxl_host::check_dims(X, Y, Z)) ||
+      (rc = need(grad_out && grad_grid && workspace, "pointers required")) ||
       (rc = need(k >= 1 && s >= 1 && k <= X && k <= Y && k <= Z, "need 1 <= k <= dims, s >= 1")))
     return rc;
-  const Dims o{(X - k) / s + 1, (Y - k) / s + 1, (Z - k) / s + 1};
-  hipLaunchKernelGGL(pxl::k_avg_pool_bwd, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
-                     (hipStream_t)stream, (const float4*)grad_out, Dims{X, Y, Z}, o, k, s,
-                     (float4*)grad_grid);
+  const int64_t ox = (X - k) / s + 1, oy = (Y - k) / s + 1, oz = (Z - k) / s + 1;
+  float4* d = reinterpret_cast<float4*>(workspace);  // (X, Y, oz)
+  float4* c = d + X * Y * oz;                         // (X, oy, oz)
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * oy * oz, 256)), dim3(256), 0, st,
+                     (const float4*)grad_out, (int64_t)1, X, oy * oz, k, s, ox, 0, c);
+  hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * Y * oz, 256)), dim3(256), 0, st,
+                     (const float4*)c, X, Y, oz, k, s, oy, 0, d);
+  hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0, st,
+                     (const float4*)d, X * Y, Z, (int64_t)1, k, s, oz, 1, (float4*)grad_grid);
   return launched("pxl_avg_pool3d_bwd");
 }
 
@@ -824,8 +883,12 @@ int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_av
   const double bc2 = 1.0 - std::pow(beta2, (double)step);
   const float neg_step = (float)(-(lr / bc1));
   const float bc2_sqrt = (float)std::pow(bc2, 0.5);
-  hipLaunchKernelGGL(pxl::k_adam, dim3(stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                     param, grad, exp_avg, exp_avg_sq, grad_abs, n, (float)(1.0 - beta1),
+  auto al16 = [](const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+  const bool vec = al16(param) && al16(grad) && al16(exp_avg) && al16(exp_avg_sq) && al16(grad_abs) &&
+                   (half_out == nullptr || (reinterpret_cast<uintptr_t>(half_out) & 7u) == 0);
+  const int64_t n4 = vec ? n / 4 : 0;
+  hipLaunchKernelGGL(pxl::k_adam, dim3(stride_blocks(n4 > 0 ? n4 : n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     param, grad, exp_avg, exp_avg_sq, grad_abs, n, n4, (float)(1.0 - beta1),
                      (float)beta2, (float)(1.0 - beta2), bc2_sqrt, (float)eps, neg_step, half_out,
                      half_dtype);
   return launched("pxl_adam_step");

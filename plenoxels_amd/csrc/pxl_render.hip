@@ -369,6 +369,40 @@ __global__ void k_dilate(const uint32_t* __restrict__ in, uint32_t X, uint32_t Y
   out[w] = word;
 }
 
+// Occupancy from the grid's own alpha: bit L set when raw alpha >= 0 at cell L (DILATE: at any
+// of the corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped -- the cells a trilinear sample with
+// floor cell L reads).  A sample whose cells all have raw alpha < 0 is exactly zero in the
+// reference's chain: clamp gives alpha 0, so its colour weight and colour gradients vanish,
+// and the clamp's gradient mask is 0 for every alpha it read.  One lane per cell, a wave ballot
+// per 64 cells (two words).  NaN alpha counts as empty.
+template <int DT, bool DILATE>
+__global__ void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X, uint32_t Y, uint32_t Z,
+                                  uint32_t* __restrict__ out, int64_t n_words) {
+  const int64_t n_cells = (int64_t)X * Y * Z;
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_cells; base += stride) {
+    const int64_t L = base + lane;
+    bool occ = false;
+    if (L < n_cells) {
+      if constexpr (DILATE) {
+        const uint32_t k = (uint32_t)(L % Z), j = (uint32_t)((L / Z) % Y), i = (uint32_t)(L / ((int64_t)Y * Z));
+        const uint32_t i1 = i + 1 == X ? 0u : i + 1, j1 = j + 1 == Y ? 0u : j + 1, k1 = k + 1 == Z ? 0u : k + 1;
+        const uint32_t is[2] = {i, i1}, js[2] = {j, j1}, ks[2] = {k, k1};
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          occ |= load_cell<DT>(grid, (is[c >> 2] * Y + js[(c >> 1) & 1]) * Z + ks[c & 1]).w >= 0.f;
+      } else {
+        occ = load_cell<DT>(grid, (uint32_t)L).w >= 0.f;
+      }
+    }
+    const uint64_t m = __ballot(occ);
+    const int64_t w = base >> 5;  // base is a multiple of 64
+    if (lane == 0) out[w] = (uint32_t)m;
+    if (lane == 32 && w + 1 < n_words) out[w + 1] = (uint32_t)(m >> 32);
+  }
+}
+
 __global__ void k_pack_occupancy(const uint8_t* __restrict__ occ, int64_t n_cells,
                                  uint32_t* __restrict__ out, int64_t n_words) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -862,6 +896,33 @@ int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z
                      (hipStream_t)stream, occ_in, (uint32_t)X, (uint32_t)Y, (uint32_t)Z, occ_out, words);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_occupancy_dilate");
+}
+
+int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid))) return rc;
+  if (!bits) return fail(PXL_E_INVALID, "bits is NULL");
+  const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
+  const int64_t words = (n + 31) / 32;
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, int64_t(1) << 20);
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
+#define PXL_ALPHA_OCC(DTV)                                                                           \
+  if (dilate)                                                                                        \
+    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, true>), dim3(nb), dim3(256), 0, s, grid->data, X, Y, Z, \
+                       bits, words);                                                                 \
+  else                                                                                               \
+    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, false>), dim3(nb), dim3(256), 0, s, grid->data, X, Y, Z, \
+                       bits, words);
+  if (grid->dtype == PXL_DTYPE_F32) {
+    PXL_ALPHA_OCC(PXL_DTYPE_F32)
+  } else if (grid->dtype == PXL_DTYPE_BF16) {
+    PXL_ALPHA_OCC(PXL_DTYPE_BF16)
+  } else {
+    PXL_ALPHA_OCC(PXL_DTYPE_F16)
+  }
+#undef PXL_ALPHA_OCC
+  return launched("pxl_alpha_occupancy");
 }
 
 int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream) {

@@ -182,25 +182,38 @@ def trilinear_interpolation(normalized_samples_for_indecies, selected_points, gr
 
 # --------------------------------------------------------------------------- grid layout, pooling
 
+def _pool_workspace(dims, k, s, device):
+    n = _lib.lib().pxl_avg_pool3d_workspace_floats(*dims, k, s)
+    if n < 0:
+        raise ValueError(f"average pooling needs 1 <= k <= dims and s >= 1 (k={k}, s={s}, dims={dims})")
+    return torch.empty(n, dtype=torch.float32, device=device)
+
+
+def pool_backward_(grad_out, dims, k, s, grad_grid):
+    """grad_grid (X,Y,Z,4) += the average-pooling backward of grad_out (pxl_avg_pool3d_bwd)."""
+    go = grad_out.float().contiguous()
+    ws = _pool_workspace(dims, k, s, go.device)
+    _lib.check(_lib.lib().pxl_avg_pool3d_bwd(_lib.ptr(go), *dims, k, s, _lib.ptr(grad_grid), _lib.ptr(ws),
+                                             _lib.stream_handle(go.device)))
+
+
 class _PoolFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, grid, k, s):
         g = _f32_grid(grid)
         X, Y, Z = g.shape[:3]
+        ws = _pool_workspace((X, Y, Z), k, s, g.device)
         out = torch.empty(((X - k) // s + 1, (Y - k) // s + 1, (Z - k) // s + 1, 4), dtype=torch.float32,
                           device=g.device)
-        _lib.check(_lib.lib().pxl_avg_pool3d(_lib.ptr(g), X, Y, Z, k, s, _lib.ptr(out),
+        _lib.check(_lib.lib().pxl_avg_pool3d(_lib.ptr(g), X, Y, Z, k, s, _lib.ptr(out), _lib.ptr(ws),
                                              _lib.stream_handle(g.device)))
         ctx.k, ctx.s, ctx.dims = k, s, (X, Y, Z)
         return out
 
     @staticmethod
     def backward(ctx, gout):
-        X, Y, Z = ctx.dims
-        grad = torch.zeros((X, Y, Z, 4), dtype=torch.float32, device=gout.device)
-        go = gout.float().contiguous()
-        _lib.check(_lib.lib().pxl_avg_pool3d_bwd(_lib.ptr(go), X, Y, Z, ctx.k, ctx.s, _lib.ptr(grad),
-                                                 _lib.stream_handle(go.device)))
+        grad = torch.zeros((*ctx.dims, 4), dtype=torch.float32, device=gout.device)
+        pool_backward_(gout, ctx.dims, ctx.k, ctx.s, grad)
         return grad, None, None
 
 

@@ -363,6 +363,19 @@ def occupancy_dilate(occ_bits, dims):
     return out
 
 
+def alpha_occupancy(grid, interp="nearest"):
+    """Occupancy bits of the cells whose raw alpha is >= 0 (trilinear: floor cells any of whose
+    8 corners is), from the grid itself (pxl_alpha_occupancy).  Skipping the other samples
+    changes no finite result of the render and training kernels: their clamped alpha is 0 and
+    the clamp's gradient mask is 0."""
+    X, Y, Z = (int(v) for v in grid.shape[:3])
+    bits = torch.empty((X * Y * Z + 31) // 32, dtype=torch.int32, device=grid.device)
+    g = _grid_struct(grid, None)
+    _lib.check(_lib.lib().pxl_alpha_occupancy(ctypes.byref(g), 1 if interp == "trilinear" else 0, _lib.ptr(bits),
+                                              _lib.stream_handle(grid.device)))
+    return bits
+
+
 def pack_occupancy(occ):
     """(X,Y,Z) bool/uint8 device tensor -> packed int32 bit words for the kernels' occupancy
     argument (bit (L & 31) of word L >> 5, L the linear cell index)."""

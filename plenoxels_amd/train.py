@@ -20,9 +20,9 @@ import torch
 
 from . import _lib
 from .checkpoint import save_grid
-from .grid_functions import average_pool3d_grid  # noqa: F401  (reference-compatible API)
+from .grid_functions import average_pool3d_grid, pool_backward_
 from .optim import adam_step_, tv_backward_, tv_sumsq
-from .render import CameraRays, CornerBlockPair, MarchConfig, beta_term_zero, fill_, train_step
+from .render import CameraRays, CornerBlockPair, MarchConfig, alpha_occupancy, beta_term_zero, fill_, train_step
 
 RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
 STEPS_PER_FIELD = 5                                              # scripts/train.py:92
@@ -54,12 +54,16 @@ class Trainer:
     corner_blocks (trilinear only) routes the gradient through per-floor-cell corner blocks
     and their reduce (pxl_train_step_blocks), which replaces the zero-fill; it costs 128 B per
     cell of device memory and a dense reduce pass per step.
+    skip_empty (default on) renders each step with the occupancy of the cells whose raw alpha
+    is >= 0 (pxl_alpha_occupancy): the samples it skips are exactly zero in the reference chain
+    (clamped alpha 0, zero clamp gradient), so finite results are unchanged while the empty
+    space the training drives below zero costs nothing.
     """
 
     def __init__(self, imgs, transform_matricies, camera_angle_x, *, gridsize, points_distance,
                  number_of_rays, num_samples, delta_step, lr, tv, beta, even_spread=False, device="cuda",
                  progressive_growing=True, interp="nearest", storage_dtype=torch.float32, uniforms_fn=None,
-                 init_grid=None, near=0.0, t_stop=0.0, corner_blocks=False):
+                 init_grid=None, near=0.0, t_stop=0.0, corner_blocks=False, skip_empty=True):
         self.dev = torch.device(device)
         self.imgs = imgs.to(self.dev).float().contiguous()
         self.c2w = transform_matricies.to(self.dev).float().contiguous()
@@ -78,6 +82,9 @@ class Trainer:
         if corner_blocks and interp != "trilinear":
             raise ValueError("corner_blocks needs interp='trilinear'")
         self.use_blocks = bool(corner_blocks)
+        self.skip_empty = bool(skip_empty)
+        if self.use_blocks and self.skip_empty:
+            raise ValueError("corner_blocks needs a fixed occupancy; use skip_empty=False with it")
         self._blocks = {}  # grid shape -> CornerBlockPair (each reduce zeroes the other buffer)
         self.B = self.imgs.shape[0]
         X, Y, Z = self.dims
@@ -136,7 +143,9 @@ class Trainer:
             if key not in self._blocks:
                 self._blocks[key] = CornerBlockPair(grid)
             pair = self._blocks[key]
-        train_step(grid, grad, cfg=cfg, camera=cam, ray_sq_err=self.sq, beta=self.beta if self.beta > 0 else 0.0,
+        occ = alpha_occupancy(grid, self.interp) if self.skip_empty else None
+        train_step(grid, grad, cfg=cfg, camera=cam, occupancy=occ, ray_sq_err=self.sq,
+                   beta=self.beta if self.beta > 0 else 0.0,
                    ray_beta=self.bsum if self.beta > 0 else None, blocks=pair.blocks if pair else None,
                    overwrite=True, clear=pair.other if pair else None)
         if pair:
@@ -159,8 +168,7 @@ class Trainer:
             loss = loss + bl
         if pooled:
             fill_(self.grad, 0.0)
-            _lib.check(_lib.lib().pxl_avg_pool3d_bwd(_lib.ptr(grad.contiguous()), X, Y, Z, k, max(1, k // 4),
-                                                     _lib.ptr(self.grad), _lib.stream_handle(self.dev)))
+            pool_backward_(grad, (X, Y, Z), k, max(1, k // 4), self.grad)
         self.step_count += 1
         adam_step_(self.grid, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count, lr=self.lr,
                    grad_abs=self.grad_abs, half_out=self.render_copy)

@@ -111,6 +111,7 @@ def main(argv=None):
     ap.add_argument("--t-stop", type=float, default=1e-4)
     ap.add_argument("--dtypes", default="bfloat16,float32")
     ap.add_argument("--corner-blocks", type=int, default=0, help="Trainer(corner_blocks=...)")
+    ap.add_argument("--skip-empty", type=int, default=1, help="Trainer(skip_empty=...)")
     ap.add_argument("--progressive", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--plain-chunk", type=int, default=4096)
@@ -128,7 +129,7 @@ def main(argv=None):
                       "rays_per_iter": (a.rays // a.views) * a.views, "samples": S, "delta": delta, "near": a.near,
                       "lr": a.lr, "tv": a.tv, "beta": 0.0, "t_stop": a.t_stop, "interp": "trilinear",
                       "progressive_growing": bool(a.progressive), "init": "colour 0.5, density 0.1",
-                      "corner_blocks": bool(a.corner_blocks)}}
+                      "corner_blocks": bool(a.corner_blocks), "skip_empty": bool(a.skip_empty)}}
 
     # ground truth
     t0 = time.time()
@@ -180,7 +181,8 @@ def main(argv=None):
         tr = Trainer(imgs_dev, c2w_train, theta, gridsize=[X] * 3, points_distance=pd, number_of_rays=R_per_cam,
                      num_samples=S, delta_step=delta, lr=a.lr, tv=a.tv, beta=0.0, device=dev,
                      progressive_growing=bool(a.progressive), interp="trilinear", storage_dtype=dt,
-                     init_grid=init.to(dev), near=a.near, t_stop=a.t_stop, corner_blocks=bool(a.corner_blocks))
+                     init_grid=init.to(dev), near=a.near, t_stop=a.t_stop, corner_blocks=bool(a.corner_blocks),
+                     skip_empty=bool(a.skip_empty))
         rgrid = lambda: tr.grid if tr.render_copy is None else tr.render_copy  # noqa: E731
         p0 = psnr(render_views_fused(rgrid(), c2w_held, theta, a.res, cfg_full, dev)[..., :3], held_dev[..., :3])
         torch.cuda.synchronize()

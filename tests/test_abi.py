@@ -137,7 +137,9 @@ def test_validation_errors_before_any_hip_call():
                             ctypes.c_void_p(0x5000), None) == _lib.PXL_E_INVALID
     d3 = _lib.dims_arg((4, 4, 0))
     assert L.pxl_wrap_indices(ctypes.c_void_p(0x10), 4, d3, None) == _lib.PXL_E_INVALID
-    assert L.pxl_avg_pool3d(ctypes.c_void_p(0x10), 4, 4, 4, 5, 1, ctypes.c_void_p(0x20), None) == _lib.PXL_E_INVALID
+    assert L.pxl_avg_pool3d(ctypes.c_void_p(0x10), 4, 4, 4, 5, 1, ctypes.c_void_p(0x20), ctypes.c_void_p(0x30), None) == _lib.PXL_E_INVALID
+    assert L.pxl_avg_pool3d_workspace_floats(4, 4, 4, 5, 1) == -1
+    assert L.pxl_avg_pool3d_workspace_floats(8, 6, 10, 3, 2) == (8 * 6 * 4 + 8 * 2 * 4) * 4
     assert L.pxl_adam_step(ctypes.c_void_p(0x10), ctypes.c_void_p(0x10), ctypes.c_void_p(0x10),
                            ctypes.c_void_p(0x10), None, 4, 1e-3, 0.9, 0.999, 1e-8, 0, None, 0,
                            None) == _lib.PXL_E_INVALID  # step counts from 1
@@ -176,6 +178,8 @@ def test_corner_block_validation_before_any_hip_call():
     g.dims[1] = 8
     assert L.pxl_blocks_reduce(ctypes.byref(g), None, ctypes.c_void_p(0x20), 1, None, None) == _lib.PXL_E_INVALID
     assert L.pxl_blocks_reduce(None, ctypes.c_void_p(0x10), ctypes.c_void_p(0x20), 1, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_alpha_occupancy(ctypes.byref(g), 1, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_alpha_occupancy(None, 1, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
 
 
 def test_no_device_is_reported_not_aborted():

@@ -145,7 +145,7 @@ def test_trainer_corner_blocks_match_direct(start, dims, progressive):
         tr = Trainer(imgs, c2w, 0.69, gridsize=dims, points_distance=0.06, number_of_rays=R, num_samples=48,
                      delta_step=0.05, lr=0.01, tv=1e-3, beta=5e-3, device="cuda", progressive_growing=progressive,
                      interp="trilinear", uniforms_fn=lambda i: uniforms[i - start], init_grid=init,
-                     corner_blocks=blocks)
+                     corner_blocks=blocks, skip_empty=False)
         hist = [{k: float(v) for k, v in tr.step(i).items()} for i in range(start, start + steps)]
         runs.append((tr, hist))
     (ta, ha), (tb, hb) = runs
