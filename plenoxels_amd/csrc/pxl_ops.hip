@@ -461,31 +461,45 @@ __device__ __forceinline__ double sq4(const float4& a, const float4& b) {
   return (double)(dx * dx) + (double)(dy * dy) + (double)(dz * dz) + (double)(dw * dw);
 }
 
+// Cell index -> (x, y, z) in 32-bit arithmetic (grids hold at most 2^31 - 1 cells).
+__device__ __forceinline__ void cell_xyz(uint32_t L, uint32_t Y, uint32_t Z, uint32_t& x, uint32_t& y,
+                                         uint32_t& z) {
+  const uint32_t r = L / Z;
+  z = L - r * Z;
+  x = r / Y;
+  y = r - x * Y;
+}
+
 __global__ __launch_bounds__(256) void k_tv_sumsq(const float4* __restrict__ g, Dims d,
                                                   double* __restrict__ sum) {
+  const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
   double acc = 0.0;
   PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
+    uint32_t x, y, z;
+    cell_xyz((uint32_t)L, Y, Z, x, y, z);
     const float4 v = g[L];
-    if (y + 1 < d.Y) acc += sq4(v, g[L + d.Z]);
-    if (z + 1 < d.Z) acc += sq4(v, g[L + 1]);
-    if (x + 1 < d.X) acc += sq4(v, g[L + d.Y * d.Z]);
+    if (y + 1 < Y) acc += sq4(v, g[L + Z]);
+    if (z + 1 < Z) acc += sq4(v, g[L + 1]);
+    if (x + 1 < X) acc += sq4(v, g[L + YZ]);
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(sum, acc);
 }
 
-__global__ void k_tv_bwd(const float4* __restrict__ g, Dims d, const double* __restrict__ sum,
-                         float weight, float* __restrict__ grad) {
+__global__ __launch_bounds__(256) void k_tv_bwd(const float4* __restrict__ g, Dims d,
+                                                const double* __restrict__ sum, float weight,
+                                                float4* __restrict__ grad) {
   // d/dx of weight*sqrt(S): weight / sqrt(S) * sum over neighbours of (x - neighbour)
   const float scale = weight / (float)sqrt(*sum);
+  const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
   PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    const int64_t z = L % d.Z, y = (L / d.Z) % d.Y, x = L / (d.Y * d.Z);
+    uint32_t x, y, z;
+    cell_xyz((uint32_t)L, Y, Z, x, y, z);
     const float4 v = g[L];
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t nb[6] = {y + 1 < d.Y ? L + d.Z : -1, y > 0 ? L - d.Z : -1,
-                           z + 1 < d.Z ? L + 1 : -1,   z > 0 ? L - 1 : -1,
-                           x + 1 < d.X ? L + d.Y * d.Z : -1, x > 0 ? L - d.Y * d.Z : -1};
+    const int64_t nb[6] = {y + 1 < Y ? L + Z : -1,  y > 0 ? L - Z : -1,
+                           z + 1 < Z ? L + 1 : -1,  z > 0 ? L - 1 : -1,
+                           x + 1 < X ? L + YZ : -1, x > 0 ? L - YZ : -1};
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if (nb[q] < 0) continue;
@@ -495,11 +509,12 @@ __global__ void k_tv_bwd(const float4* __restrict__ g, Dims d, const double* __r
       acc.z += v.z - u.z;
       acc.w += v.w - u.w;
     }
-    float* p = grad + 4 * L;
-    p[0] += scale * acc.x;
-    p[1] += scale * acc.y;
-    p[2] += scale * acc.z;
-    p[3] += scale * acc.w;
+    float4 c = grad[L];
+    c.x += scale * acc.x;
+    c.y += scale * acc.y;
+    c.z += scale * acc.z;
+    c.w += scale * acc.w;
+    grad[L] = c;
   }
 }
 
@@ -860,11 +875,13 @@ int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double*
                float weight, float* grad_grid, void* stream) {
   int rc;
   if ((rc = pxl_host::check_dims(X, Y, Z)) ||
-      (rc = need(grid && sum_sq && grad_grid, "grid, sum_sq and grad_grid required")))
+      (rc = need(grid && sum_sq && grad_grid, "grid, sum_sq and grad_grid required")) ||
+      (rc = need(((reinterpret_cast<uintptr_t>(grid) | reinterpret_cast<uintptr_t>(grad_grid)) & 15u) == 0,
+                 "grid and grad_grid must be 16-byte aligned")))
     return rc;
   hipLaunchKernelGGL(pxl::k_tv_bwd, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
                      (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq, weight,
-                     grad_grid);
+                     reinterpret_cast<float4*>(grad_grid));
   return launched("pxl_tv_bwd");
 }
 
