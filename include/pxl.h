@@ -141,30 +141,32 @@ int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
                    const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream);
 
 /* pxl_train_step with the trilinear gradient accumulated into corner blocks instead of
- * grad_grid: blocks is (X*Y*Z, 32) f32 (pxl_blocks_floats), zero on entry, and receives for
- * each kept sample's floor cell F its eight corner gradients (block F, float 4*j + c holds
- * channel c of corner j = 4bx + 2by + bz, b = 0 the ceil index and b = 1 the floor index, the
- * corner order of src/grid_functions.py:238-243), clip mask applied.  A sample's 32 atomics
- * then fall in one 128-byte block (two memory requests instead of about five for the cells
- * themselves).  pxl_blocks_reduce turns the blocks into the (X,Y,Z,4) gradient.  Several
- * calls may accumulate into the same blocks before one reduce.  interp must be trilinear;
- * at most 2^29 - 1 cells.  Same replacement as pxl_train_step. */
+ * grad_grid: blocks is (X*Y*Z, 16) f32 (pxl_blocks_floats), zero on entry: one 64-byte entry
+ * per cell E = (x, y, z), whose slot s = 2bx + by (floats 4s + c) receives channel c of corner
+ * (bx, by, bz) -- b = 0 the ceil index, b = 1 the floor index, the corner order of
+ * src/grid_functions.py:238-243 -- of every kept sample whose floor cell is (x, y, zf) and
+ * whose corner z (zf for bz = 1, the wrapped ceil of zf for bz = 0) is z, clip mask applied.
+ * A sample's 32 atomics then fall in two 64-byte entries (two memory requests instead of about
+ * five for the cells themselves).  pxl_blocks_reduce turns the blocks into the (X,Y,Z,4)
+ * gradient.  Several calls may accumulate into the same blocks before one reduce.  interp must
+ * be trilinear; at most 2^30 - 1 cells.  Same replacement as pxl_train_step. */
 int pxl_train_step_blocks(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
                           const pxl_loss* loss, float* out_rgba, float* blocks, void* stream);
 
-/* Floats of the corner blocks of a grid (X*Y*Z*32), or -1 for invalid dims. */
+/* Floats of the corner blocks of a grid (X*Y*Z*16), or -1 for invalid dims. */
 int64_t pxl_blocks_floats(const pxl_grid* grid);
 
-/* grad_grid (X,Y,Z,4) f32 = (accumulate: +=) the sum, for every cell, of the corner entries
- * that name it (corner j of floor cell F names F + (1-bx, 1-by, 1-bz) wrapped, the ceil of
- * the last index wrapping to 0 as src/grid_functions.py:168 does).  Only blocks of occupied
- * floor cells are read when grid->occupancy is set (pass the occupancy the training steps
- * used); grid->data is not read.  clear: blocks to zero for the next training step --
- * == blocks: these blocks, in a second pass after the sum; another buffer of the same size:
- * its blocks, in the same pass (double buffering: the next step accumulates into `clear`
- * while this one's blocks wait for their own clearing pass); NULL: none.  Only blocks of
- * occupied floor cells are zeroed when occupancy is set, so keep the occupancy of a block
- * buffer fixed or zero it whole when it changes. */
+/* grad_grid (X,Y,Z,4) f32 = (accumulate: +=) the sum, for every cell (x, y, z), of slot
+ * 2bx + by of entry (x - 1 + bx, y - 1 + by, z) over bx, by in {0, 1} (corner (bx, by) of row
+ * (x, y) names cell (x + 1 - bx, y + 1 - by), the ceil of the last index wrapping to 0 as
+ * src/grid_functions.py:168 does).  When grid->occupancy is set (pass the occupancy the
+ * training steps used) only entries that such steps can write are read: those of cells whose
+ * bit, or whose z - 1 neighbour's bit (wrapped), is set; grid->data is not read.  clear:
+ * blocks to zero for the next training step -- == blocks: these blocks, in a second pass after
+ * the sum; another buffer of the same size: its entries, in the same pass (double buffering:
+ * the next step accumulates into `clear` while this one's blocks wait for their own clearing
+ * pass); NULL: none.  Only those same entries are zeroed when occupancy is set, so keep the
+ * occupancy of a block buffer fixed or zero it whole when it changes. */
 int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int32_t accumulate,
                       float* clear, void* stream);
 

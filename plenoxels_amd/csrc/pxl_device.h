@@ -183,7 +183,7 @@ struct KParams {
   float gx, gy, gz;
   int exact_div;
   int aux_stride;
-  int corner_blocks;  // trilinear gradients go to per-floor-cell corner blocks (32 floats)
+  int corner_blocks;  // trilinear gradients go to per-cell corner-block entries (16 floats)
   int64_t n_rays;
   int64_t ray_offset;
   // rays
@@ -433,8 +433,9 @@ struct ScatterItems {
 
 // Per-sample gradient items: the cells sample k read and d loss / d cell value for each
 // (clip-gradient mask applied), written to LDS at slot `base`.  The item's index is the cell
-// (the flush adds into grad + 4*index), or, with corner blocks, floor_cell*8 + corner, so that
-// the sample's eight items land in one contiguous 128-byte block.
+// (the flush adds into grad + 4*index), or, with corner blocks, entry*4 + slot: the entry is
+// the floor cell's (x, y) row at the corner's z (floor z, or the wrapped ceil z), the slot
+// 2bx + by, so the sample's eight items fill two 64-byte entries (adjacent unless z wraps).
 template <int INTERP>
 __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int k, uint32_t clip,
                                             const float4& gv, uint32_t* s_cell, float4* s_val,
@@ -451,7 +452,7 @@ __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int 
     TriCell t;
     trilinear_cell(P, nx, ny, nz, t);
     const float ofx = 1.f - t.fx, ofy = 1.f - t.fy, ofz = 1.f - t.fz;
-    const uint32_t block = (t.xs[1] + t.ys[1] + t.zs[1]) * 8u;
+    const uint32_t row = t.xs[1] + t.ys[1];  // floor (x, y) row of the corner-block entries
 #pragma unroll
     for (int bz = 0; bz < 2; ++bz) {
       const float wz = bz == 0 ? t.fz : ofz;
@@ -465,7 +466,8 @@ __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int 
           const float wx = bx == 0 ? t.fx : ofx;
           const int j = 4 * bx + 2 * by + bz;
           const uint32_t cb = clip >> (4 * j);
-          s_cell[base + j] = P.corner_blocks ? block + (uint32_t)j : t.xs[bx] + t.ys[by] + t.zs[bz];
+          s_cell[base + j] = P.corner_blocks ? (row + t.zs[bz]) * 4u + (uint32_t)(2 * bx + by)
+                                             : t.xs[bx] + t.ys[by] + t.zs[bz];
           s_val[base + j] = make_float4((cb & 1u) ? gy.x * wx : 0.f, (cb & 2u) ? gy.y * wx : 0.f,
                                         (cb & 4u) ? gy.z * wx : 0.f, (cb & 8u) ? gy.w * wx : 0.f);
         }

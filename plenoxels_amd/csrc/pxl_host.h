@@ -12,8 +12,8 @@
 namespace pxl_host {
 
 constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
-// corner blocks are indexed floor_cell * 8 + corner in 32 bits
-constexpr int64_t kMaxBlockCells = (int64_t(1) << 29) - 1;
+// corner-block entries are indexed entry * 4 + slot in 32 bits
+constexpr int64_t kMaxBlockCells = (int64_t(1) << 30) - 1;
 
 // Records the thread-local error message returned by pxl_last_error(); returns code.
 PXL_HIDDEN const char* last_error_text();

@@ -442,22 +442,22 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
   out[384 + lane] = wave_shift_down(x, -1.f);
 }
 
-// Corner blocks -> gradient.  Block F (8 float4) holds, for corner j = 4bx + 2by + bz of
-// floor cell F (b = 0: the ceil index, F + 1 wrapped; b = 1: F itself), the four channel
-// gradients.  Output cell c therefore collects corner j from F = c - (1-bx, 1-by, 1-bz),
-// wrapped, summed in the fixed order j = 0..7; blocks whose floor cell is unoccupied were never
-// written and are skipped.
+// Corner blocks -> gradient.  Entry E = (x, y, z) (4 float4) holds in slot s = 2bx + by the
+// four channel gradients of corner (bx, by) of the samples whose floor cell is (x, y, *) and
+// whose corner z is z (b = 0: the ceil index, wrapped; b = 1: the floor index).  Corner (bx, by)
+// of row (x, y) names cell (x + 1 - bx, y + 1 - by), so output cell (x, y, z) sums slot m of
+// entry (x - 1 + bx, y - 1 + by, z), wrapped, for m = 2bx + by = 0..3 in that order.  Entry
+// (x, y, z) is written only by samples with floor cell (x, y, z) or (x, y, z - 1 wrapped), so
+// with occupancy the entries of cells where neither bit is set are skipped.
 //
 // A workgroup covers up to 256 cells of one (x, y) row, one thread per cell along z; its four
-// waves work independently (no LDS, no barrier).  The four source rows m = 2bx + by are
+// waves work independently (no LDS, no barrier).  The four source rows m are
 // (x-1+bx, y-1+by).  Each lane tests its own z in each source row (the z-1 bits come from the
-// lane below by a DPP shift) and reads, from the block of its own z in source row m, the
-// adjacent corner pair j = 2m, 2m+1 (32 bytes, one memory segment): corner 2m+1 (bz = 1) is its
-// own, corner 2m (bz = 0) belongs to cell z+1 and moves one lane up by a DPP shift; the first
-// lane of each wave reads its z-1 bits and corners itself.  Rows are numbered so that the
-// workgroups in flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the
-// same blocks.  `clear` (a second block buffer, not read in this pass) has the blocks of each
-// wave's occupied cells zeroed in the same pass by coalesced stores, ready for the next step.
+// lane below by a DPP shift; the first lane of each wave reads its own) and reads slot m of the
+// entry of its own z in source row m (16 bytes).  Rows are numbered so that the workgroups in
+// flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the same entries.
+// `clear` (a second block buffer, not read in this pass) has the written entries of each wave's
+// own row zeroed in the same pass by coalesced stores, ready for the next step.
 constexpr int kReduceCells = 256;
 
 __device__ __forceinline__ uint32_t shift_up_u32(uint32_t v, uint32_t fill) {
@@ -493,35 +493,21 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
       if (active && occupied(occ, B[m] + z)) own |= 1u << m;
       if (lane == 0 && active && occupied(occ, B[m] + zp)) own_p |= 1u << m;
     }
-    const uint32_t prev = shift_up_u32(own, own_p);  // bits of (m, z-1)
-    // all block loads first (independent, in flight together), then the shifts and the sum
-    float4 lo[4], hi[4], halo[4];  // corners 2m (bz = 0), 2m+1 (bz = 1); lane 0: corner 2m at z-1
+    // entry (m, z) may hold sums; lanes past the row's end (z >= Z) touch nothing
+    const uint32_t written = active ? own | shift_up_u32(own, own_p) : 0u;
+    float4 v[4];  // all loads first (independent, in flight together), then the sum
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      lo[m] = hi[m] = halo[m] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if ((own >> m) & 1u) {
-        const float4* b = blocks + (size_t)(B[m] + z) * 8u + 2 * m;
-        lo[m] = b[0];
-        hi[m] = b[1];
-      }
-      if (lane == 0 && ((prev >> m) & 1u)) halo[m] = blocks[(size_t)(B[m] + zp) * 8u + 2 * m];
+      v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if ((written >> m) & 1u) v[m] = blocks[(size_t)(B[m] + z) * 4u + m];
     }
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 acc = v[0];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      float4 up;  // corner 2m of the block at z-1: lane l-1's lo, or lane 0's own halo load
-      up.x = wave_shift_up(lo[m].x, halo[m].x);
-      up.y = wave_shift_up(lo[m].y, halo[m].y);
-      up.z = wave_shift_up(lo[m].z, halo[m].z);
-      up.w = wave_shift_up(lo[m].w, halo[m].w);
-      acc.x += up.x;
-      acc.y += up.y;
-      acc.z += up.z;
-      acc.w += up.w;
-      acc.x += hi[m].x;
-      acc.y += hi[m].y;
-      acc.z += hi[m].z;
-      acc.w += hi[m].w;
+    for (int m = 1; m < 4; ++m) {
+      acc.x += v[m].x;
+      acc.y += v[m].y;
+      acc.z += v[m].z;
+      acc.w += v[m].w;
     }
     if (active) {
       const uint32_t c = B[3] + z;
@@ -531,28 +517,33 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
       }
       grad[c] = acc;
     }
-    if (clear) {  // this wave's 64 cells = 512 float4: lane l of pass k clears float4 64k + l
+    if (clear) {  // this wave's 64 entries = 256 float4: lane l of pass k clears float4 64k + l
       const uint32_t wz0 = z0 + (uint32_t)(tid & ~63);
-      float4* cb = clear + (size_t)(B[3] + wz0) * 8u;
-      const uint32_t own3 = (own >> 3) & 1u;
+      float4* cb = clear + (size_t)(B[3] + wz0) * 4u;
+      const uint32_t w3 = (written >> 3) & 1u;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int cell = 8 * k + (lane >> 3);  // lane within the wave whose cell this is
-        if (__shfl(own3, cell)) cb[64 * k + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int k = 0; k < 4; ++k) {
+        const int cell = 16 * k + (lane >> 2);  // lane within the wave whose entry this is
+        if (__shfl(w3, cell)) cb[64 * k + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
   }
 }
 
-// Zero every block of an occupied floor cell (every block without occupancy).  Eight lanes per
-// 128-byte block, so each store instruction covers eight whole consecutive blocks.
+// Zero every entry that a step with this occupancy may have written (entry (x, y, z) when the
+// bit of (x, y, z) or (x, y, z - 1 wrapped) is set; every entry without occupancy).  Four lanes
+// per 64-byte entry, so each store instruction covers sixteen whole consecutive entries.
 __global__ __launch_bounds__(256) void k_blocks_zero(float4* __restrict__ blocks,
                                                      const uint32_t* __restrict__ occ,
-                                                     int64_t n_cells) {
-  const int64_t n = n_cells * 8;
+                                                     int Z, int64_t n_cells) {
+  const int64_t n = n_cells * 4;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
-    if (occupied(occ, (uint32_t)(t >> 3))) blocks[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const uint32_t e = (uint32_t)(t >> 2);
+    const uint32_t ez = e % (uint32_t)Z;
+    const uint32_t ep = ez == 0 ? e + (uint32_t)Z - 1u : e - 1u;
+    if (occupied(occ, e) || occupied(occ, ep)) blocks[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 }  // namespace pxl
@@ -796,7 +787,7 @@ static int train_step(const pxl_grid* grid, const pxl_march* march, const pxl_ra
   if (blocks && march->interp != PXL_INTERP_TRILINEAR)
     return fail(PXL_E_INVALID, "corner blocks hold trilinear gradients (interp must be trilinear)");
   if (blocks && grid->dims[0] * grid->dims[1] * grid->dims[2] > kMaxBlockCells)
-    return fail(PXL_E_SIZE, "corner blocks index floor_cell*8 + corner in 32 bits: at most %lld cells",
+    return fail(PXL_E_SIZE, "corner blocks index entry*4 + slot in 32 bits: at most %lld cells",
                 (long long)kMaxBlockCells);
   if (!std::isfinite(loss->grad_scale) || !std::isfinite(loss->beta_scale))
     return fail(PXL_E_INVALID, "grad_scale and beta_scale must be finite");
@@ -842,7 +833,7 @@ int pxl_train_step_blocks(const pxl_grid* grid, const pxl_march* march, const px
 
 int64_t pxl_blocks_floats(const pxl_grid* grid) {
   if (!grid || grid->dims[0] <= 0 || grid->dims[1] <= 0 || grid->dims[2] <= 0) return -1;
-  return grid->dims[0] * grid->dims[1] * grid->dims[2] * 32;
+  return grid->dims[0] * grid->dims[1] * grid->dims[2] * 16;
 }
 
 int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int32_t accumulate,
@@ -853,7 +844,7 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
   if (!blocks || !grad_grid) return fail(PXL_E_INVALID, "blocks and grad_grid are required");
   const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
   if (n > kMaxBlockCells)
-    return fail(PXL_E_SIZE, "corner blocks index floor_cell*8 + corner in 32 bits: at most %lld cells",
+    return fail(PXL_E_SIZE, "corner blocks index entry*4 + slot in 32 bits: at most %lld cells",
                 (long long)kMaxBlockCells);
   hipStream_t s = (hipStream_t)stream;
   const int64_t units = grid->dims[0] * grid->dims[1] * ((grid->dims[2] + pxl::kReduceCells - 1) / pxl::kReduceCells);
@@ -866,8 +857,8 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
                      reinterpret_cast<float4*>(grad_grid));
   if ((rc = launched("pxl_blocks_reduce"))) return rc;
   if (!separate) return PXL_OK;
-  hipLaunchKernelGGL(pxl::k_blocks_zero, dim3(stride_blocks(n * 8, 256)), dim3(256), 0, s,
-                     reinterpret_cast<float4*>(blocks), grid->occupancy, n);
+  hipLaunchKernelGGL(pxl::k_blocks_zero, dim3(stride_blocks(n * 4, 256)), dim3(256), 0, s,
+                     reinterpret_cast<float4*>(blocks), grid->occupancy, (int)grid->dims[2], n);
   return launched("pxl_blocks_reduce");
 }
 

@@ -206,10 +206,11 @@ def camera_rays(camera, want_targets=True):
 
 
 def corner_blocks(grid):
-    """Zeroed corner blocks for `grid` ((X*Y*Z, 32) float32 on its device): the accumulation
-    buffer of train_step(..., blocks=...), kept zero between steps by blocks_reduce."""
+    """Zeroed corner blocks for `grid` ((X*Y*Z, 16) float32 on its device, one 64-byte entry
+    per cell): the accumulation buffer of train_step(..., blocks=...), kept zero between
+    steps by blocks_reduce."""
     X, Y, Z = (int(v) for v in grid.shape[:3])
-    return torch.zeros((X * Y * Z, 32), dtype=torch.float32, device=grid.device)
+    return torch.zeros((X * Y * Z, 16), dtype=torch.float32, device=grid.device)
 
 
 class CornerBlockPair:
@@ -249,9 +250,9 @@ def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clea
 
 def _check_blocks(grid, blocks, grad_grid):
     X, Y, Z = (int(v) for v in grid.shape[:3])
-    if (blocks.dtype != torch.float32 or not blocks.is_contiguous() or blocks.numel() != X * Y * Z * 32
+    if (blocks.dtype != torch.float32 or not blocks.is_contiguous() or blocks.numel() != X * Y * Z * 16
             or blocks.device != grid.device):
-        raise ValueError("blocks must be contiguous float32 with X*Y*Z*32 elements on the grid's device")
+        raise ValueError("blocks must be contiguous float32 with X*Y*Z*16 elements on the grid's device")
     if grad_grid is not None and (grad_grid.dtype != torch.float32 or tuple(grad_grid.shape) != (X, Y, Z, 4)
                                   or not grad_grid.is_contiguous()):
         raise ValueError("grad_grid must be a contiguous (X,Y,Z,4) float32 tensor")

@@ -51,9 +51,9 @@ class Trainer:
     uniforms_fn(step) -> (B,R,2) uniforms overrides torch.rand (tests feed fixed ones);
     init_grid replaces the all-zero initial grid; near (first sample at near + delta) and
     t_stop (early ray termination) are the march extensions, both 0 in the reference.
-    corner_blocks (trilinear only) routes the gradient through per-floor-cell corner blocks
-    and their reduce (pxl_train_step_blocks), which replaces the zero-fill; it costs 128 B per
-    cell of device memory and a dense reduce pass per step.
+    corner_blocks (trilinear only) routes the gradient through corner blocks (a 64-byte entry
+    per cell) and their reduce (pxl_train_step_blocks), which replaces the zero-fill; it costs
+    two such buffers, 128 B per cell of device memory, and a dense reduce pass per step.
     skip_empty (default on) renders each step with the occupancy of the cells whose raw alpha
     is >= 0 (pxl_alpha_occupancy): the samples it skips are exactly zero in the reference chain
     (clamped alpha 0, zero clamp gradient), so finite results are unchanged while the empty

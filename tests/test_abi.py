@@ -166,12 +166,12 @@ def test_corner_block_validation_before_any_hip_call():
     assert L.pxl_train_step_blocks(*args) == _lib.PXL_E_INVALID  # nearest scatters one cell per sample
     assert "trilinear" in _lib.last_error()
     m.interp = _lib.INTERP_TRILINEAR
-    g.dims[0] = g.dims[1] = g.dims[2] = 1024  # 2^30 cells > 2^29 - 1
+    g.dims[0] = g.dims[1] = g.dims[2] = 1024  # 2^30 cells > 2^30 - 1
     assert L.pxl_train_step_blocks(*args) == _lib.PXL_E_SIZE
     g.dims[0] = g.dims[1] = g.dims[2] = 8
     assert L.pxl_train_step_blocks(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss), None,
                                    None, None) == _lib.PXL_E_INVALID
-    assert L.pxl_blocks_floats(ctypes.byref(g)) == 8 * 8 * 8 * 32
+    assert L.pxl_blocks_floats(ctypes.byref(g)) == 8 * 8 * 8 * 16
     g.dims[1] = 0
     assert L.pxl_blocks_floats(ctypes.byref(g)) == -1
     assert L.pxl_blocks_reduce(ctypes.byref(g), ctypes.c_void_p(0x10), ctypes.c_void_p(0x20), 1, None, None) == _lib.PXL_E_INVALID
