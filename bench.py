@@ -292,7 +292,7 @@ def main():
         achieved = b_alg / (kern_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args),
-                    "kernel": "k_train_step", "kernel_ms": round(kern_ms, 4),
+                    "kernel": "k_train_pair", "kernel_ms": round(kern_ms, 4),
                     "algorithmic_bytes": b_alg}
         extra.update({"mean_processed_samples_per_ray": round(mean_proc, 2),
                       "mean_distinct_cells_per_ray": round(mean_cells, 2)})

@@ -1,7 +1,8 @@
 // pxl_render.hip -- fused render / backward / training-step kernels for gfx950 and the
 // C ABI declared in include/pxl.h.
 //
-// Work mapping: one 64-lane wavefront per ray, four rays per 256-thread block.
+// Work mapping: one 64-lane wavefront per ray, four rays per 256-thread block (the train step:
+// two rays per wavefront, sharing one group of 64 lanes when each keeps at most 32 samples).
 //   classify   every sample k = 64c + lane + 1 is placed and tested (in grid, occupied) in
 //              chunks of 64; kept samples are compacted, in order, into a per-wave LDS list
 //   groups     the expensive work (cell loads, clip, interpolation, compositing, scatter)
@@ -241,38 +242,15 @@ struct LossParams {
   float* beta;
 };
 
+// The train step of one ray after classification: its kept samples (list[0..nk)) in groups of
+// 64, loss and its gradient, the reverse sweep and the scatter.  `list` may alias the staging
+// area (s_cell, s_val): it is read completely before the first scatter.
 template <int DT, int INTERP, int NG>
-__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
-                                                        float* __restrict__ out,
-                                                        float* __restrict__ grad) {
-  const float* targets = LP.targets;
-  using L = WaveLds<INTERP, kWave * NG>;
-  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
-  const int lane = threadIdx.x & 63;
-  const int64_t ray = wave_ray(P);
-  if (ray >= P.n_rays) return;
-  unsigned char* mem = s_mem[threadIdx.x >> 6];
-  int* list = L::list(mem);
-  Ray R;
-  setup_ray(P, ray, R);
-  const float4 tgt = targets ? reinterpret_cast<const float4*>(targets)[ray] : camera_target(P, R);
+__device__ __forceinline__ void train_ray(const KParams& P, const LossParams& LP, const Ray& R,
+                                          int64_t ray, const float4& tgt, const int* list, int nk,
+                                          int lane, uint32_t* s_cell, float4* s_val,
+                                          float* __restrict__ out, float* __restrict__ grad) {
   const int S = P.n_samples;
-  // classify all NG chunks before the first ballot (their occupancy loads overlap), then
-  // compact the kept samples in order
-  bool kp[NG];
-#pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    const int k = (c << 6) + lane + 1;
-    kp[c] = classify<INTERP>(P, R, k) & (k <= S);
-  }
-  int nk = 0;
-#pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    const uint64_t km = __ballot(kp[c]);
-    if (kp[c]) list[nk + lane_rank(km)] = (c << 6) + lane + 1;
-    nk += __popcll(km);
-  }
-  wave_lds_sync();
   const int ng = (nk + 63) >> 6;
   float4 V[NG];
   uint32_t CL[NG];
@@ -307,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
       T = T * readlane(incl, 63);
     }
   }
-  wave_lds_sync();  // the list area becomes the scatter staging area
+  wave_lds_sync();  // the list area may become the scatter staging area
   cr = wave_sum(cr);
   cg = wave_sum(cg);
   cb = wave_sum(cb);
@@ -328,8 +306,230 @@ __global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
     if (g < ng && KEEP[g] != 0ull) {
       const bool keep = (KEEP[g] >> lane) & 1ull;
       reverse_chunk<INTERP>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, LP.beta_scale, Qn,
-                            grad, L::cells(mem), L::vals(mem));
+                            grad, s_cell, s_val);
     }
+  }
+}
+
+template <int DT, int INTERP, int NG>
+__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
+                                                        float* __restrict__ out,
+                                                        float* __restrict__ grad) {
+  const float* targets = LP.targets;
+  using L = WaveLds<INTERP, kWave * NG>;
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  unsigned char* mem = s_mem[threadIdx.x >> 6];
+  int* list = L::list(mem);
+  Ray R;
+  setup_ray(P, ray, R);
+  const float4 tgt = targets ? reinterpret_cast<const float4*>(targets)[ray] : camera_target(P, R);
+  const int S = P.n_samples;
+  // classify all NG chunks before the first ballot (their occupancy loads overlap), then
+  // compact the kept samples in order
+  bool kp[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    const int k = (c << 6) + lane + 1;
+    kp[c] = classify<INTERP>(P, R, k) & (k <= S);
+  }
+  int nk = 0;
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    const uint64_t km = __ballot(kp[c]);
+    if (kp[c]) list[nk + lane_rank(km)] = (c << 6) + lane + 1;
+    nk += __popcll(km);
+  }
+  wave_lds_sync();
+  train_ray<DT, INTERP, NG>(P, LP, R, ray, tgt, list, nk, lane, L::cells(mem), L::vals(mem), out, grad);
+}
+
+// Two rays per wavefront.  Both rays are classified together (the occupancy loads of all their
+// chunks in flight at once); when each keeps at most 32 samples (most rays of sparse scenes)
+// ray 0 runs in lanes 0-31 and ray 1 in lanes 32-63 of one group: the row-wise scans and sums
+// stop before the step that joins the two halves (half_prefix_mul, half_suffix_affine,
+// half_sums), which evaluates, for each half, the same operation tree the one-ray kernel uses
+// on lanes 0-31 -- so every value is bit-identical to k_train_step's.  Otherwise the two rays
+// run one after the other through the one-ray path.
+__device__ __forceinline__ float half_prefix_mul(float x) {
+  x *= dpp<DPP_ROW_SHR1>(1.f, x);
+  x *= dpp<DPP_ROW_SHR2>(1.f, x);
+  x *= dpp<DPP_ROW_SHR4>(1.f, x);
+  x *= dpp<DPP_ROW_SHR8>(1.f, x);
+  x *= dpp<DPP_ROW_BCAST15, 0xa>(1.f, x);
+  return x;
+}
+
+// Row-wise partial sums up to the row-15 broadcast: lane 31 holds the sum of lanes 0-31 and
+// lane 63 that of lanes 32-63, in wave_sum's order.
+__device__ __forceinline__ float half_sums(float x) {
+  x += dpp<DPP_ROW_SHR1>(0.f, x);
+  x += dpp<DPP_ROW_SHR2>(0.f, x);
+  x += dpp<DPP_ROW_SHR4>(0.f, x);
+  x += dpp<DPP_ROW_SHR8>(0.f, x);
+  x += dpp<DPP_ROW_BCAST15, 0xa>(0.f, x);
+  return x;
+}
+
+// wave_suffix_affine within each 32-lane half (rows 0-1 and rows 2-3).
+__device__ __forceinline__ void half_suffix_affine(Affine4& f, int lane) {
+  affine_step<DPP_ROW_SHL1>(f);
+  affine_step<DPP_ROW_SHL2>(f);
+  affine_step<DPP_ROW_SHL4>(f);
+  affine_step<DPP_ROW_SHL8>(f);
+  const Affine4 r1 = affine_readlane(f, 16), r3 = affine_readlane(f, 48);
+  const int row = lane >> 4;
+  if (row == 0) f = affine_compose(f, r1);
+  if (row == 2) f = affine_compose(f, r3);
+}
+
+__device__ __forceinline__ Ray select_ray(bool second, const Ray& a, const Ray& b) {
+  Ray r;
+  r.ox = second ? b.ox : a.ox;
+  r.oy = second ? b.oy : a.oy;
+  r.oz = second ? b.oz : a.oz;
+  r.dx = second ? b.dx : a.dx;
+  r.dy = second ? b.dy : a.dy;
+  r.dz = second ? b.dz : a.dz;
+  r.cam = second ? b.cam : a.cam;
+  r.u = second ? b.u : a.u;
+  r.v = second ? b.v : a.v;
+  return r;
+}
+
+// Rays ray0 (lanes 0-31, list0[0..n0)) and ray0 + 1 (lanes 32-63, list1[0..n1)), n0, n1 <= 32,
+// one group of each; the steps of train_ray with T = 1 and Qn = 0.
+template <int DT, int INTERP>
+__device__ __forceinline__ void train_half_pair(const KParams& P, const LossParams& LP,
+                                                const Ray& R0, const Ray& R1, int64_t ray0,
+                                                bool has1, const float4& tgt0, const float4& tgt1,
+                                                const int* list0, int n0, const int* list1, int n1,
+                                                int lane, uint32_t* s_cell, float4* s_val,
+                                                float* __restrict__ out, float* __restrict__ grad) {
+  const bool hi = lane >= 32;
+  const int h = lane & 31;
+  const bool valid = h < (hi ? n1 : n0);
+  const int k = valid ? (hi ? list1[h] : list0[h]) : 0;
+  const Ray R = select_ray(hi, R0, R1);
+  float4 V = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t CL = 0u;
+  if (valid) eval_kept<DT, INTERP>(P, R, k, V, CL);
+  const float incl = half_prefix_mul(1.f - V.w);
+  // lane 32 starts ray 1 with transmittance 1: the boundary value goes into the source lane
+  // (31) before the shift, so every lane uses the DPP result.  (Selecting on the shifted value
+  // instead lets the compiler run the DPP under a lane mask, and a DPP read of a disabled
+  // lane returns the fill value.)
+  const float Ti = wave_shift_up(lane == 31 ? 1.f : incl, 1.f);
+  bool proc = valid;
+  if (P.t_stop > 0.f) {  // stop_check per half
+    const uint64_t dead = __ballot(valid && Ti < P.t_stop);
+    const uint32_t dh = hi ? (uint32_t)(dead >> 32) : (uint32_t)dead;
+    if (dh) proc = proc && h < __builtin_ctz(dh);
+  }
+  float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f, bsum = 0.f;
+  accumulate(proc, Ti, V, cr, cg, cb, ca);
+  if (LP.beta && proc) bsum += beta_term(V.w) - beta_term(0.f);
+  const uint64_t keep_mask = __ballot(proc);
+  wave_lds_sync();  // list0 becomes the scatter staging area
+  cr = half_sums(cr);
+  cg = half_sums(cg);
+  cb = half_sums(cb);
+  ca = half_sums(ca);
+  if (LP.beta) bsum = half_sums(bsum);
+  // each ray's totals and loss gradient, formed in the lane that writes its outputs (31, 63)
+  const float d0 = cr - (hi ? tgt1.x : tgt0.x), d1 = cg - (hi ? tgt1.y : tgt0.y);
+  const float d2 = cb - (hi ? tgt1.z : tgt0.z), d3 = ca - (hi ? tgt1.w : tgt0.w);
+  if (h == 31 && (!hi || has1)) {
+    const int64_t ray = ray0 + (hi ? 1 : 0);
+    if (out) reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
+    if (LP.sq_err) LP.sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+    if (LP.beta) LP.beta[ray] = bsum;
+  }
+  if (keep_mask == 0ull) return;
+  const float gs = LP.grad_scale;
+  const float4 gl0 = make_float4(gs * readlane(d0, 31), gs * readlane(d1, 31), gs * readlane(d2, 31),
+                                 gs * readlane(d3, 31));
+  const float4 gl1 = make_float4(gs * readlane(d0, 63), gs * readlane(d1, 63), gs * readlane(d2, 63),
+                                 gs * readlane(d3, 63));
+  const float4 gl = hi ? gl1 : gl0;
+  // reverse sweep of the one group (reverse_chunk with Qn = 0 and the halves kept apart)
+  const float a = proc ? V.w : 0.f;
+  Affine4 f;
+  f.a0 = a * V.x;
+  f.a1 = a * V.y;
+  f.a2 = a * V.z;
+  f.a3 = a;
+  f.b = 1.f - a;
+  half_suffix_affine(f, lane);
+  // what lies behind each sample: the next lane's suffix, nothing behind lane 31 (end of ray 0)
+  // or lane 63; lane 32 feeds 0 to lane 31 for the same reason as above
+  const bool first1 = lane == 32;
+  const float s0 = wave_shift_down(first1 ? 0.f : f.a0, 0.f), s1 = wave_shift_down(first1 ? 0.f : f.a1, 0.f);
+  const float s2 = wave_shift_down(first1 ? 0.f : f.a2, 0.f), s3 = wave_shift_down(first1 ? 0.f : f.a3, 0.f);
+  float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (proc) {
+    const float w = a * Ti;
+    gv.x = w * gl.x;
+    gv.y = w * gl.y;
+    gv.z = w * gl.z;
+    gv.w = Ti * (gl.x * (V.x - s0) + gl.y * (V.y - s1) + gl.z * (V.z - s2) + gl.w * (1.f - s3));
+    if (LP.beta_scale != 0.f) gv.w += LP.beta_scale * beta_dloss(V.w);
+  }
+  scatter_chunk<INTERP>(P, R, lane, k, proc, CL, gv, grad, s_cell, s_val);
+}
+
+template <int DT, int INTERP, int NG>
+__global__ __launch_bounds__(kBlock) void k_train_pair(KParams P, LossParams LP,
+                                                        float* __restrict__ out,
+                                                        float* __restrict__ grad) {
+  // per wave: [ray 0's list, then the scatter staging area | ray 1's list]
+  using L = WaveLds<INTERP, kWave * NG>;
+  constexpr int kList1 = kWave * NG * 4;
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes + kList1];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t ray0 = P.ray_offset + 2 * ((int64_t)blockIdx.x * kWavesPerBlock + wave);
+  if (ray0 >= P.n_rays) return;
+  const bool has1 = ray0 + 1 < P.n_rays;
+  unsigned char* mem = s_mem[wave];
+  int* list0 = L::list(mem);
+  int* list1 = reinterpret_cast<int*>(mem + L::kBytes);
+  Ray R0, R1;
+  setup_ray(P, ray0, R0);
+  if (has1) setup_ray(P, ray0 + 1, R1);
+  else R1 = R0;
+  const float* targets = LP.targets;
+  const float4 tgt0 = targets ? reinterpret_cast<const float4*>(targets)[ray0] : camera_target(P, R0);
+  const float4 tgt1 = !has1 ? tgt0
+                    : targets ? reinterpret_cast<const float4*>(targets)[ray0 + 1] : camera_target(P, R1);
+  const int S = P.n_samples;
+  bool kp0[NG], kp1[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    const int k = (c << 6) + lane + 1;
+    kp0[c] = classify<INTERP>(P, R0, k) & (k <= S);
+    kp1[c] = classify<INTERP>(P, R1, k) & (k <= S) & has1;
+  }
+  int n0 = 0, n1 = 0;
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    const uint64_t km0 = __ballot(kp0[c]), km1 = __ballot(kp1[c]);
+    if (kp0[c]) list0[n0 + lane_rank(km0)] = (c << 6) + lane + 1;
+    if (kp1[c]) list1[n1 + lane_rank(km1)] = (c << 6) + lane + 1;
+    n0 += __popcll(km0);
+    n1 += __popcll(km1);
+  }
+  wave_lds_sync();
+  if (n0 <= 32 && n1 <= 32) {
+    train_half_pair<DT, INTERP>(P, LP, R0, R1, ray0, has1, tgt0, tgt1, list0, n0, list1, n1, lane,
+                                L::cells(mem), L::vals(mem), out, grad);
+  } else {
+    train_ray<DT, INTERP, NG>(P, LP, R0, ray0, tgt0, list0, n0, lane, L::cells(mem), L::vals(mem), out, grad);
+    if (has1)
+      train_ray<DT, INTERP, NG>(P, LP, R1, ray0 + 1, tgt1, list1, n1, lane, L::cells(mem), L::vals(mem), out,
+                                grad);
   }
 }
 
@@ -610,7 +810,7 @@ using pxl_host::check_dims;
 using pxl_host::launched;
 using pxl_host::stride_blocks;
 
-constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // 2^24 blocks of 4 rays
+constexpr int64_t kMaxRaysPerLaunch = int64_t(1) << 26;  // at most 2^24 blocks per launch
 
 int check_march(const pxl_march* m) {
   if (!m) return fail(PXL_E_INVALID, "march descriptor is NULL");
@@ -802,19 +1002,20 @@ static int train_step(const pxl_grid* grid, const pxl_march* march, const pxl_ra
   hipStream_t s = (hipStream_t)stream;
   const int S = march->n_samples;
   PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
-    rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+    // two rays per wavefront (k_train_pair), eight per block
+    rc = for_ray_pieces(P, 2 * pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
       const pxl::LossParams LQ = LP;  // per-ray arrays are indexed by the global ray number
       if (S <= 64)
-        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 1>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+        hipLaunchKernelGGL((pxl::k_train_pair<kDT, kIN, 1>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
                            LQ, out_rgba, grad_grid);
       else if (S <= 128)
-        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 2>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+        hipLaunchKernelGGL((pxl::k_train_pair<kDT, kIN, 2>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
                            LQ, out_rgba, grad_grid);
       else if (S <= 256)
-        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 4>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+        hipLaunchKernelGGL((pxl::k_train_pair<kDT, kIN, 4>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
                            LQ, out_rgba, grad_grid);
       else
-        hipLaunchKernelGGL((pxl::k_train_step<kDT, kIN, 8>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
+        hipLaunchKernelGGL((pxl::k_train_pair<kDT, kIN, 8>), dim3(nb), dim3(pxl::kBlock), 0, s, Q,
                            LQ, out_rgba, grad_grid);
     }, blocks ? "pxl_train_step_blocks" : "pxl_train_step");
   });
