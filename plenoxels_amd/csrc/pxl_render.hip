@@ -650,81 +650,106 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
 // (x, y, z) is written only by samples with floor cell (x, y, z) or (x, y, z - 1 wrapped), so
 // with occupancy the entries of cells where neither bit is set are skipped.
 //
-// A workgroup covers up to 256 cells of one (x, y) row, one thread per cell along z; its four
-// waves work independently (no LDS, no barrier).  The four source rows m are
-// (x-1+bx, y-1+by).  Each lane tests its own z in each source row (the z-1 bits come from the
-// lane below by a DPP shift; the first lane of each wave reads its own) and reads slot m of the
-// entry of its own z in source row m (16 bytes).  Rows are numbered so that the workgroups in
-// flight on one XCD (workgroup id mod 8) hold neighbouring y rows, which read the same entries.
-// `clear` (a second block buffer, not read in this pass) has the written entries of each wave's
-// own row zeroed in the same pass by coalesced stores, ready for the next step.
+// A workgroup covers up to 256 cells of NY consecutive (x, y) rows, one thread per cell along z
+// (a lane handles its z in all NY rows); its four waves work independently (no LDS, no
+// barrier).  Output row y0 + i reads slot m = 2bx + by from source row (x - 1 + bx, y0 - 1 + i
+// + by), so the NY rows share NY + 1 source rows per bx: each lane tests its own z in each
+// (the z - 1 bits come from the lane below by a DPP shift; the first lane of each wave reads its
+// own), then issues all 4 NY entry loads (16 bytes each) before summing, so one wave keeps
+// NY times the loads in flight behind a single occupancy round trip.  Units are numbered so
+// that the workgroups in flight on one XCD (workgroup id mod 8) hold neighbouring y rows,
+// which read the same entries.  `clear` (a second block buffer, not read in this pass) has the
+// written entries of each output row zeroed in the same pass by coalesced stores, ready for
+// the next step.
 constexpr int kReduceCells = 256;
+constexpr int kReduceRows = 4;  // NY
 
 __device__ __forceinline__ uint32_t shift_up_u32(uint32_t v, uint32_t fill) {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, DPP_WAVE_SHR1, 0xf, 0xf, false);
 }
 
+template <int NY>
 __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __restrict__ blocks,
                                                                 const uint32_t* __restrict__ occ,
                                                                 int X, int Y, int Z, int accumulate,
                                                                 float4* __restrict__ clear,
                                                                 float4* __restrict__ grad) {
+  constexpr int NR = NY + 1;  // source rows per bx
   const uint32_t nzc = ((uint32_t)Z + kReduceCells - 1) / kReduceCells;
-  const uint32_t n_units = (uint32_t)X * (uint32_t)Y * nzc;
+  const uint32_t nyg = ((uint32_t)Y + NY - 1) / NY;
+  const uint32_t n_units = (uint32_t)X * nyg * nzc;
   const uint32_t n_wg = ((n_units + 7u) / 8u) * 8u, per_xcd = n_wg / 8u;
   const int tid = (int)threadIdx.x, lane = tid & 63;
   for (uint32_t wg = blockIdx.x; wg < n_wg; wg += gridDim.x) {
     const uint32_t u = (wg % 8u) * per_xcd + wg / 8u;  // XCD-contiguous unit order
     if (u >= n_units) continue;
     const uint32_t zc = u % nzc, rxy = u / nzc;
-    const uint32_t x = rxy / (uint32_t)Y, y = rxy % (uint32_t)Y;
+    const uint32_t x = rxy / nyg, y0 = (rxy % nyg) * NY;
     const uint32_t z0 = zc * kReduceCells;
     const uint32_t nz = min((uint32_t)kReduceCells, (uint32_t)Z - z0);
-    const uint32_t xm = x == 0 ? (uint32_t)X - 1u : x - 1u, ym = y == 0 ? (uint32_t)Y - 1u : y - 1u;
-    const uint32_t B[4] = {(xm * (uint32_t)Y + ym) * (uint32_t)Z, (xm * (uint32_t)Y + y) * (uint32_t)Z,
-                           (x * (uint32_t)Y + ym) * (uint32_t)Z, (x * (uint32_t)Y + y) * (uint32_t)Z};
+    const uint32_t xm = x == 0 ? (uint32_t)X - 1u : x - 1u;
+    // source row r (0..NY) is y0 - 1 + r (r = 0 wraps at y0 = 0); rows past Y - 1 are unused
+    uint32_t base[2][NR];
+    bool rv[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t yr = r == 0 ? (y0 == 0 ? (uint32_t)Y - 1u : y0 - 1u) : y0 - 1u + (uint32_t)r;
+      rv[r] = r == 0 || yr < (uint32_t)Y;
+      base[0][r] = (xm * (uint32_t)Y + yr) * (uint32_t)Z;
+      base[1][r] = (x * (uint32_t)Y + yr) * (uint32_t)Z;
+    }
     const uint32_t z = z0 + (uint32_t)tid;
     const bool active = (uint32_t)tid < nz;
     const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
-    // occupancy bits: own[m] for (m, z); the lane below supplies (m, z-1), lane 0 loads it
+    // occupancy bit (bx, r) at bit bx * NR + r: own for z; the lane below supplies z - 1
     uint32_t own = 0u, own_p = 0u;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      if (active && occupied(occ, B[m] + z)) own |= 1u << m;
-      if (lane == 0 && active && occupied(occ, B[m] + zp)) own_p |= 1u << m;
-    }
-    // entry (m, z) may hold sums; lanes past the row's end (z >= Z) touch nothing
-    const uint32_t written = active ? own | shift_up_u32(own, own_p) : 0u;
-    float4 v[4];  // all loads first (independent, in flight together), then the sum
+    for (int bx = 0; bx < 2; ++bx)
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if ((written >> m) & 1u) v[m] = blocks[(size_t)(B[m] + z) * 4u + m];
-    }
-    float4 acc = v[0];
-#pragma unroll
-    for (int m = 1; m < 4; ++m) {
-      acc.x += v[m].x;
-      acc.y += v[m].y;
-      acc.z += v[m].z;
-      acc.w += v[m].w;
-    }
-    if (active) {
-      const uint32_t c = B[3] + z;
-      if (accumulate) {
-        const float4 g = grad[c];
-        acc = make_float4(g.x + acc.x, g.y + acc.y, g.z + acc.z, g.w + acc.w);
+      for (int r = 0; r < NR; ++r) {
+        if (active && rv[r] && occupied(occ, base[bx][r] + z)) own |= 1u << (bx * NR + r);
+        if (lane == 0 && active && rv[r] && occupied(occ, base[bx][r] + zp)) own_p |= 1u << (bx * NR + r);
       }
-      grad[c] = acc;
-    }
-    if (clear) {  // this wave's 64 entries = 256 float4: lane l of pass k clears float4 64k + l
-      const uint32_t wz0 = z0 + (uint32_t)(tid & ~63);
-      float4* cb = clear + (size_t)(B[3] + wz0) * 4u;
-      const uint32_t w3 = (written >> 3) & 1u;
+    // entry (bx, r, z) may hold sums; lanes past the row's end (z >= Z) touch nothing
+    const uint32_t written = active ? own | shift_up_u32(own, own_p) : 0u;
+    float4 v[NY][4];  // all loads first (independent, in flight together), then the sums
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int cell = 16 * k + (lane >> 2);  // lane within the wave whose entry this is
-        if (__shfl(w3, cell)) cb[64 * k + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 0; i < NY; ++i)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int bx = m >> 1, r = i + (m & 1);
+        v[i][m] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (y0 + (uint32_t)i < (uint32_t)Y && ((written >> (bx * NR + r)) & 1u))
+          v[i][m] = blocks[(size_t)(base[bx][r] + z) * 4u + m];
+      }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) {
+      if (y0 + (uint32_t)i >= (uint32_t)Y) break;
+      float4 acc = v[i][0];
+#pragma unroll
+      for (int m = 1; m < 4; ++m) {
+        acc.x += v[i][m].x;
+        acc.y += v[i][m].y;
+        acc.z += v[i][m].z;
+        acc.w += v[i][m].w;
+      }
+      if (active) {
+        const uint32_t c = base[1][i + 1] + z;
+        if (accumulate) {
+          const float4 g = grad[c];
+          acc = make_float4(g.x + acc.x, g.y + acc.y, g.z + acc.z, g.w + acc.w);
+        }
+        grad[c] = acc;
+      }
+      if (clear) {  // row i's 64 entries of this wave = 256 float4: lane l of pass k clears 64k + l
+        const uint32_t wz0 = z0 + (uint32_t)(tid & ~63);
+        float4* cb = clear + (size_t)(base[1][i + 1] + wz0) * 4u;
+        const uint32_t wr = (written >> (NR + i + 1)) & 1u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int cell = 16 * k + (lane >> 2);  // lane within the wave whose entry this is
+          if (__shfl(wr, cell)) cb[64 * k + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
       }
     }
   }
@@ -1048,10 +1073,11 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
     return fail(PXL_E_SIZE, "corner blocks index entry*4 + slot in 32 bits: at most %lld cells",
                 (long long)kMaxBlockCells);
   hipStream_t s = (hipStream_t)stream;
-  const int64_t units = grid->dims[0] * grid->dims[1] * ((grid->dims[2] + pxl::kReduceCells - 1) / pxl::kReduceCells);
+  const int64_t units = grid->dims[0] * ((grid->dims[1] + pxl::kReduceRows - 1) / pxl::kReduceRows) *
+                        ((grid->dims[2] + pxl::kReduceCells - 1) / pxl::kReduceCells);
   const unsigned n_wg = (unsigned)std::min<int64_t>(((units + 7) / 8) * 8, int64_t(1) << 24);
   const bool separate = clear == blocks;  // the pass reads `blocks`: clear them afterwards
-  hipLaunchKernelGGL(pxl::k_blocks_reduce, dim3(n_wg), dim3(pxl::kReduceCells), 0, s,
+  hipLaunchKernelGGL(pxl::k_blocks_reduce<pxl::kReduceRows>, dim3(n_wg), dim3(pxl::kReduceCells), 0, s,
                      reinterpret_cast<const float4*>(blocks), grid->occupancy, (int)grid->dims[0],
                      (int)grid->dims[1], (int)grid->dims[2], accumulate ? 1 : 0,
                      separate ? nullptr : reinterpret_cast<float4*>(clear),
