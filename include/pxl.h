@@ -56,10 +56,14 @@ typedef struct pxl_grid {
   int64_t dims[3];           /* X, Y, Z */
   int32_t channels;          /* must be 4 */
   int32_t dtype;             /* PXL_DTYPE_* */
-  const uint32_t* occupancy; /* optional: bit (L & 31) of word L >> 5 set = cell L occupied.
-                                Nearest skips a sample whose nearest cell is unoccupied;
-                                trilinear skips one whose floor cell is unoccupied (see
-                                pxl_occupancy_dilate).  NULL = every cell occupied. */
+  const uint32_t* occupancy; /* optional occupancy bits in 8x8x8 tiles (pxl_occupancy_words
+                                words): tile (x>>3, y>>3, z>>3) is the 16 words from
+                                ((tx*TY + ty)*TZ + tz)*16, TY = ceil(Y/8), TZ = ceil(Z/8), and
+                                cell (x,y,z) is bit (y&3)*8 + (z&7) of its word (x&7)*2 +
+                                ((y>>2)&1); set = occupied.  Nearest skips a sample whose
+                                nearest cell is unoccupied; trilinear skips one whose floor
+                                cell is unoccupied (see pxl_occupancy_dilate).  NULL = every
+                                cell occupied. */
 } pxl_grid;
 
 /* How samples are placed along each ray and mapped to grid-index space
@@ -175,22 +179,29 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
 int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,
                     void* stream);
 
-/* Cell occupancy for trilinear skipping: out bit L = OR of in bits of the 8 corners
- * (i+a, j+b, k+c), a,b,c in {0,1}, wrapped.  out must not alias in; words = ceil(XYZ/32). */
+/* Cell occupancy for trilinear skipping: the out bit of cell (i,j,k) = OR of the in bits of
+ * its 8 corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped.  Both in the tile layout of
+ * pxl_grid.occupancy (pxl_occupancy_words words); out must not alias in. */
 int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z,
                          uint32_t* occ_out, void* stream);
 
-/* Occupancy from the grid's alpha channel: bit L of bits (ceil(XYZ/32) words, the layout of
- * pxl_grid.occupancy) set when raw alpha >= 0 at cell L, or, with dilate, at any of its 8
+/* Occupancy from the grid's alpha channel: the bit of cell L in bits (pxl_occupancy_words
+ * words, the layout of pxl_grid.occupancy) set when raw alpha >= 0 at cell L, or, with
+ * dilate, at any of its 8
  * corners (the cells a trilinear sample with floor cell L reads).  Samples that this skips are
  * exactly zero in the reference chain (clamp gives alpha 0 and a zero gradient mask), so
  * passing it to the render kernels changes no finite result; NaN alpha counts as empty.
  * grid->occupancy is not read. */
 int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, void* stream);
 
-/* Pack a (X,Y,Z) occupancy grid of bytes (nonzero = occupied) into the bit layout of
- * pxl_grid.occupancy: bit (L & 31) of word L >> 5.  out: ceil(n_cells/32) words. */
-int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream);
+/* Pack a (X,Y,Z) occupancy grid of bytes (nonzero = occupied) into the tile layout of
+ * pxl_grid.occupancy.  out: pxl_occupancy_words(X, Y, Z) words (padding bits 0). */
+int pxl_pack_occupancy(const uint8_t* occ, int64_t X, int64_t Y, int64_t Z, uint32_t* out,
+                       void* stream);
+
+/* Words of the occupancy bits of an (X,Y,Z) grid: 16 per 8x8x8 tile, dims rounded up to a
+ * multiple of 8; -1 for invalid dims. */
+int64_t pxl_occupancy_words(int64_t X, int64_t Y, int64_t Z);
 
 /* dst[0..n) = value (HBM-rate fill, e.g. the per-step gradient zero-fill). */
 int pxl_fill_f32(float* dst, int64_t n, float value, void* stream);

@@ -292,12 +292,18 @@ def mse_loss(pred, target):
 # --------------------------------------------------------------------------- occupancy
 
 def occupancy_to_bits(occ):
-    """(X,Y,Z) bool -> packed uint32 words, bit (L & 31) of word L >> 5, L = (i*Y+j)*Z+k."""
-    flat = occ.reshape(-1).to(torch.uint8).cpu().numpy()
-    pad = (-flat.size) % 32
-    flat = np.concatenate([flat, np.zeros(pad, np.uint8)])
-    bits = np.packbits(flat.reshape(-1, 32)[:, ::-1], axis=1, bitorder="big")
-    words = bits.view(">u4").astype(np.uint32).reshape(-1)
+    """(X,Y,Z) bool -> int32 words in the kernels' occupancy layout: 8x8x8 tiles of 16 words,
+    tile ((x>>3)*TY + (y>>3))*TZ + (z>>3) with TY = ceil(Y/8), TZ = ceil(Z/8); cell (x,y,z) is
+    bit (y&3)*8 + (z&7) of word (x&7)*2 + ((y>>2)&1) of its tile (padding cells 0)."""
+    o = occ.to(torch.bool).cpu().numpy()
+    X, Y, Z = o.shape
+    TX, TY, TZ = (X + 7) // 8, (Y + 7) // 8, (Z + 7) // 8
+    pad = np.zeros((TX * 8, TY * 8, TZ * 8), dtype=bool)
+    pad[:X, :Y, :Z] = o
+    # axes (tx, xi, ty, yhi, ylo, tz, zi) -> word (tx, ty, tz, xi, yhi), bit (ylo, zi)
+    t = pad.reshape(TX, 8, TY, 2, 4, TZ, 8).transpose(0, 2, 5, 1, 3, 4, 6)
+    bits = t.reshape(-1, 32).astype(np.uint64)
+    words = (bits << np.arange(32, dtype=np.uint64)).sum(axis=1).astype(np.uint32)
     return torch.from_numpy(words.view(np.int32).copy())
 
 

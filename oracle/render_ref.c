@@ -45,9 +45,15 @@ typedef struct {
   int ncorner;
 } ref_sample;
 
-static int occ_bit(const uint32_t* occ, int64_t cell) {
+/* Occupancy bit of cell (x, y, z): 8x8x8 tiles of 16 words, tile ((x>>3)*TY + (y>>3))*TZ +
+ * (z>>3) (TY = ceil(Y/8), TZ = ceil(Z/8)), bit (y&3)*8 + (z&7) of its word (x&7)*2 +
+ * ((y>>2)&1) -- the layout the kernels read. */
+static int occ_bit(const ref_params* p, const uint32_t* occ, int64_t x, int64_t y, int64_t z) {
   if (!occ) return 1;
-  return (int)((occ[cell >> 5] >> (cell & 31)) & 1u);
+  const int64_t TY = (p->Y + 7) / 8, TZ = (p->Z + 7) / 8;
+  const int64_t t = ((x >> 3) * TY + (y >> 3)) * TZ + (z >> 3);
+  const int64_t w = t * 16 + (x & 7) * 2 + ((y >> 2) & 1);
+  return (int)((occ[w] >> (((y & 3) << 3) | (z & 7))) & 1u);
 }
 
 static int64_t wrap(int64_t i, int64_t n) {
@@ -86,7 +92,7 @@ static void eval_sample(const ref_params* p, const double* grid, const uint32_t*
     }
     if (!inb) return;
     int64_t cell = (idx[0] * p->Y + idx[1]) * p->Z + idx[2];
-    if (!occ_bit(occ, cell)) return;
+    if (!occ_bit(p, occ, idx[0], idx[1], idx[2])) return;
     s->keep = 1;
     s->ncorner = 1;
     s->cell[0] = cell;
@@ -105,7 +111,7 @@ static void eval_sample(const ref_params* p, const double* grid, const uint32_t*
     fr[a] = (double)(n[a] - truncf(n[a]));
   }
   int64_t floor_cell = (cs[0][1] * p->Y + cs[1][1]) * p->Z + cs[2][1];
-  if (!occ_bit(occ, floor_cell)) return;
+  if (!occ_bit(p, occ, cs[0][1], cs[1][1], cs[2][1])) return;
   s->keep = 1;
   s->ncorner = 8;
   int j = 0;

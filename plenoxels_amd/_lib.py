@@ -72,7 +72,9 @@ SIGNATURES = {
     "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, c_void_p, c_void_p]),
-    "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                          c_void_p, c_void_p]),
+    "pxl_occupancy_words": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "pxl_alpha_occupancy": (ctypes.c_int, [_GP, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_fill_f32": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p]),
     "pxl_sample_points": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64,

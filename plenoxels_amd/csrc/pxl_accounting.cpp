@@ -16,8 +16,13 @@
 
 namespace {
 
-bool occ_bit(const uint32_t* occ, uint32_t cell) {
-  return occ == nullptr || ((occ[cell >> 5] >> (cell & 31u)) & 1u);
+// Occupancy bit of cell (x, y, z) in the 8x8x8-tile layout of pxl_grid.occupancy.
+bool occ_bit(const uint32_t* occ, int64_t x, int64_t y, int64_t z, int64_t Y, int64_t Z) {
+  if (occ == nullptr) return true;
+  const int64_t TY = (Y + 7) / 8, TZ = (Z + 7) / 8;
+  const int64_t t = ((x >> 3) * TY + (y >> 3)) * TZ + (z >> 3);
+  const int64_t w = t * 16 + (x & 7) * 2 + ((y >> 2) & 1);
+  return (occ[w] >> (((y & 3) << 3) | (z & 7))) & 1u;
 }
 
 void count_range(const pxl_grid* g, const pxl_march* m, const float* o, const float* d,
@@ -42,7 +47,7 @@ void count_range(const pxl_grid* g, const pxl_march* m, const float* o, const fl
         const float rx = std::nearbyint(n[0]), ry = std::nearbyint(n[1]), rz = std::nearbyint(n[2]);
         if (!(rx >= 0.f && rx < fX && ry >= 0.f && ry < fY && rz >= 0.f && rz < fZ)) continue;
         const uint32_t cell = (uint32_t)(((int64_t)rx * Y + (int64_t)ry) * Z + (int64_t)rz);
-        if (occ_bit(g->occupancy, cell)) ids.push_back(cell);
+        if (occ_bit(g->occupancy, (int64_t)rx, (int64_t)ry, (int64_t)rz, Y, Z)) ids.push_back(cell);
       } else {
         if (!(n[0] >= 0.f && n[0] < fX && n[1] >= 0.f && n[1] < fY && n[2] >= 0.f && n[2] < fZ)) continue;
         int64_t lo[3], hi[3];
@@ -52,8 +57,7 @@ void count_range(const pxl_grid* g, const pxl_march* m, const float* o, const fl
           hi[a] = (int64_t)std::ceil(n[a]);
           if (hi[a] >= dims[a]) hi[a] -= dims[a];
         }
-        const uint32_t floor_cell = (uint32_t)((lo[0] * Y + lo[1]) * Z + lo[2]);
-        if (!occ_bit(g->occupancy, floor_cell)) continue;
+        if (!occ_bit(g->occupancy, lo[0], lo[1], lo[2], Y, Z)) continue;
         for (int j = 0; j < 8; ++j) {
           const int64_t x = (j & 4) ? lo[0] : hi[0];
           const int64_t y = (j & 2) ? lo[1] : hi[1];

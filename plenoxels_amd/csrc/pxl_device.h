@@ -168,8 +168,39 @@ __device__ __forceinline__ uint32_t clip_bits(const float4& v) {
          ((uint32_t)(v.z >= 0.f && v.z <= 1.f) << 2) | ((uint32_t)(v.w >= 0.f && v.w <= 1.f) << 3);
 }
 
-__device__ __forceinline__ bool occupied(const uint32_t* occ, uint32_t cell) {
-  return occ == nullptr || ((occ[cell >> 5] >> (cell & 31u)) & 1u);
+// ----------------------------------------------------------------------------- occupancy
+// Occupancy bits are stored in 8x8x8 tiles of 512 bits, one 64-byte line (16 words) each:
+// tile (x>>3, y>>3, z>>3) holds words ((tx*TY + ty)*TZ + tz)*16 .. +15 (TY = ceil(Y/8),
+// TZ = ceil(Z/8)), and cell (x, y, z) is bit (y&3)*8 + (z&7) of word (x&7)*2 + ((y>>2)&1) of
+// its tile.  The samples of a ray, in any direction, stay in one tile for several steps, so
+// the occupancy loads of a chunk of 64 samples touch ~12 lines instead of ~34 with a linear
+// bit order (sparse256 rays).
+struct OccBit {
+  uint32_t word, shift;
+};
+
+__device__ __forceinline__ OccBit occ_bit(uint32_t x, uint32_t y, uint32_t z, uint32_t TY,
+                                          uint32_t TZ) {
+  const uint32_t t = ((x >> 3) * TY + (y >> 3)) * TZ + (z >> 3);
+  return OccBit{t * 16u + ((x & 7u) << 1) + ((y >> 2) & 1u), ((y & 3u) << 3) | (z & 7u)};
+}
+
+__device__ __forceinline__ bool occupied_at(const uint32_t* occ, uint32_t x, uint32_t y, uint32_t z,
+                                            uint32_t TY, uint32_t TZ) {
+  if (occ == nullptr) return true;
+  const OccBit b = occ_bit(x, y, z, TY, TZ);
+  return ((occ[b.word] >> b.shift) & 1u) != 0u;
+}
+
+// Cell (x, y, z) of occupancy word w, bit b (the inverse of occ_bit); coordinates may lie in
+// the padding of a partial tile (>= the grid's dims).
+__device__ __forceinline__ void occ_cell(uint32_t w, uint32_t b, uint32_t TY, uint32_t TZ,
+                                         uint32_t& x, uint32_t& y, uint32_t& z) {
+  const uint32_t t = w >> 4, sub = w & 15u;
+  const uint32_t tz = t % TZ, ty = (t / TZ) % TY, tx = t / (TZ * TY);
+  x = tx * 8u + (sub >> 1);
+  y = ty * 8u + (sub & 1u) * 4u + (b >> 3);
+  z = tz * 8u + (b & 7u);
 }
 
 // ----------------------------------------------------------------------------- parameters
@@ -184,6 +215,7 @@ struct KParams {
   int exact_div;
   int aux_stride;
   int corner_blocks;  // trilinear gradients go to per-cell corner-block entries (16 floats)
+  uint32_t occ_ty, occ_tz;  // occupancy tiles along y and z (ceil(Y/8), ceil(Z/8))
   int64_t n_rays;
   int64_t ray_offset;
   // rays
@@ -274,27 +306,39 @@ __device__ __forceinline__ void setup_ray(const KParams& P, int64_t r, Ray& R) {
 
 // ----------------------------------------------------------------------------- geometry
 
-__device__ __forceinline__ float to_index(const KParams& P, float p, float g) {
+// (p - g) / points_distance: a true division (EXACT) or a multiply by the rounded reciprocal.
+template <bool EXACT>
+__device__ __forceinline__ float to_index_t(const KParams& P, float p, float g) {
   const float diff = p - g;
-  return P.exact_div ? __fdiv_rn(diff, P.pd) : diff * P.inv_pd;
+  return EXACT ? __fdiv_rn(diff, P.pd) : diff * P.inv_pd;
 }
 
 // Continuous grid-index coordinate of sample k (src/ray_sampling.py:161-167, :13).
-__device__ __forceinline__ void sample_index(const KParams& P, const Ray& R, int k, float& nx,
-                                             float& ny, float& nz) {
+template <bool EXACT>
+__device__ __forceinline__ void sample_index_t(const KParams& P, const Ray& R, int k, float& nx,
+                                               float& ny, float& nz) {
   float t = P.delta * (float)k;
   t = t + P.near;
-  nx = to_index(P, R.ox + R.dx * t, P.gx);
-  ny = to_index(P, R.oy + R.dy * t, P.gy);
-  nz = to_index(P, R.oz + R.dz * t, P.gz);
+  nx = to_index_t<EXACT>(P, R.ox + R.dx * t, P.gx);
+  ny = to_index_t<EXACT>(P, R.oy + R.dy * t, P.gy);
+  nz = to_index_t<EXACT>(P, R.oz + R.dz * t, P.gz);
+}
+
+__device__ __forceinline__ void sample_index(const KParams& P, const Ray& R, int k, float& nx,
+                                             float& ny, float& nz) {
+  if (P.exact_div) sample_index_t<true>(P, R, k, nx, ny, nz);
+  else sample_index_t<false>(P, R, k, nx, ny, nz);
 }
 
 // Nearest voxel: round half to even, integer in-bounds test (src/grid_functions.py:111-112).
 __device__ __forceinline__ bool nearest_cell(const KParams& P, float nx, float ny, float nz,
                                              uint32_t& cell) {
   const float rx = __builtin_rintf(nx), ry = __builtin_rintf(ny), rz = __builtin_rintf(nz);
-  const bool inb = rx >= 0.f && rx < P.fX && ry >= 0.f && ry < P.fY && rz >= 0.f && rz < P.fZ;
-  cell = inb ? ((uint32_t)rx * (uint32_t)P.Y + (uint32_t)ry) * (uint32_t)P.Z + (uint32_t)rz : 0u;
+  // branch-free: the six tests combined with '&', the index computed for every lane and
+  // selected (a lane outside the grid, or with a NaN coordinate, gets cell 0)
+  const bool inb = (rx >= 0.f) & (rx < P.fX) & (ry >= 0.f) & (ry < P.fY) & (rz >= 0.f) & (rz < P.fZ);
+  const uint32_t c = ((uint32_t)rx * (uint32_t)P.Y + (uint32_t)ry) * (uint32_t)P.Z + (uint32_t)rz;
+  cell = inb ? c : 0u;
   return inb;
 }
 
@@ -329,30 +373,79 @@ __device__ __forceinline__ bool trilinear_cell(const KParams& P, float nx, float
 
 // ----------------------------------------------------------------------------- samples
 
-// Floor cell of an in-grid trilinear sample (the cell its occupancy bit is read from).
-__device__ __forceinline__ bool trilinear_floor_cell(const KParams& P, float nx, float ny,
-                                                     float nz, uint32_t& cell) {
-  const bool inb = nx >= 0.f && nx < P.fX && ny >= 0.f && ny < P.fY && nz >= 0.f && nz < P.fZ;
-  cell = inb ? ((uint32_t)floorf(nx) * (uint32_t)P.Y + (uint32_t)floorf(ny)) * (uint32_t)P.Z +
-                   (uint32_t)floorf(nz)
-             : 0u;
+// Whether sample k lies in the grid (nearest: its rounded index, src/grid_functions.py:111-112;
+// trilinear: the unrounded coordinate, :162), and the occupancy bit that decides it (nearest:
+// its cell; trilinear: its floor cell); bit 0 of word 0 when it lies outside.  Branch-free:
+// the tests are combined with '&' and the bit is computed for every lane and selected.
+template <int INTERP, bool EXACT>
+__device__ __forceinline__ bool locate(const KParams& P, const Ray& R, int k, OccBit& b) {
+  float nx, ny, nz;
+  sample_index_t<EXACT>(P, R, k, nx, ny, nz);
+  float fx, fy, fz;
+  bool inb;
+  if constexpr (INTERP == PXL_INTERP_NEAREST) {
+    fx = __builtin_rintf(nx);
+    fy = __builtin_rintf(ny);
+    fz = __builtin_rintf(nz);
+    inb = (fx >= 0.f) & (fx < P.fX) & (fy >= 0.f) & (fy < P.fY) & (fz >= 0.f) & (fz < P.fZ);
+  } else {
+    inb = (nx >= 0.f) & (nx < P.fX) & (ny >= 0.f) & (ny < P.fY) & (nz >= 0.f) & (nz < P.fZ);
+    fx = floorf(nx);
+    fy = floorf(ny);
+    fz = floorf(nz);
+  }
+  const OccBit o = occ_bit((uint32_t)fx, (uint32_t)fy, (uint32_t)fz, P.occ_ty, P.occ_tz);
+  b.word = inb ? o.word : 0u;
+  b.shift = inb ? o.shift : 0u;
   return inb;
 }
 
 // True when sample k lies in the grid and is not skipped by occupancy: the samples whose
 // values the reference chain can make nonzero.  Geometry only -- no grid value is read.
-// Branch-free: every lane loads an occupancy word (word 0 for a sample outside the grid), so
-// the loads of several chunks classified back to back are in flight together.
+// Every lane loads an occupancy word (word 0 for a sample outside the grid).
 template <int INTERP>
 __device__ __forceinline__ bool classify(const KParams& P, const Ray& R, int k) {
-  float nx, ny, nz;
-  sample_index(P, R, k, nx, ny, nz);
-  uint32_t cell;
-  const bool inb = INTERP == PXL_INTERP_NEAREST ? nearest_cell(P, nx, ny, nz, cell)
-                                                : trilinear_floor_cell(P, nx, ny, nz, cell);
+  OccBit b;
+  const bool inb = P.exact_div ? locate<INTERP, true>(P, R, k, b) : locate<INTERP, false>(P, R, k, b);
   if (P.occ == nullptr) return inb;
-  const uint32_t word = P.occ[cell >> 5];
-  return inb & (((word >> (cell & 31u)) & 1u) != 0u);
+  return inb & (((P.occ[b.word] >> b.shift) & 1u) != 0u);
+}
+
+// classify() for NC chunks of each of NR rays (sample k = 64c + lane + 1 of ray r): every sample
+// is placed first (the division mode chosen once, outside the loops), then all occupancy words
+// are loaded back to back and only then tested, so the loads of all chunks share one memory
+// round trip.  kp[r][c]: in the grid, occupied and k <= n_samples.
+template <int INTERP, int NR, int NC>
+__device__ __forceinline__ void classify_chunks(const KParams& P, const Ray (&R)[NR], int lane,
+                                                bool (&kp)[NR][NC]) {
+  OccBit ob[NR][NC];
+  bool inb[NR][NC];
+  if (P.exact_div) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) inb[r][c] = locate<INTERP, true>(P, R[r], (c << 6) + lane + 1, ob[r][c]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) inb[r][c] = locate<INTERP, false>(P, R[r], (c << 6) + lane + 1, ob[r][c]);
+  }
+  if (P.occ != nullptr) {
+    uint32_t w[NR][NC];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) w[r][c] = P.occ[ob[r][c].word];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) inb[r][c] = inb[r][c] & (((w[r][c] >> ob[r][c].shift) & 1u) != 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) kp[r][c] = inb[r][c] & ((c << 6) + lane + 1 <= P.n_samples);
 }
 
 // torch.clamp(v, 0, 1): NaN stays NaN (v_med3 alone would return a bound)

@@ -24,6 +24,8 @@ PXL_HIDDEN int launched(const char* what);
 PXL_HIDDEN int check_grid(const pxl_grid* g);
 // dims positive and X*Y*Z within the kernels' 31-bit cell index.
 PXL_HIDDEN int check_dims(int64_t X, int64_t Y, int64_t Z);
+PXL_HIDDEN int64_t occupancy_words(int64_t X, int64_t Y, int64_t Z);
+PXL_HIDDEN int check_occupancy_words(int64_t X, int64_t Y, int64_t Z);
 
 // Blocks for a grid-stride loop over n items with `threads` threads per block.
 inline unsigned stride_blocks(int64_t n, int threads, int64_t cap = 256 * 64) {

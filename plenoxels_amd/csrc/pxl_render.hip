@@ -311,48 +311,15 @@ __device__ __forceinline__ void train_ray(const KParams& P, const LossParams& LP
   }
 }
 
-template <int DT, int INTERP, int NG>
-__global__ __launch_bounds__(kBlock) void k_train_step(KParams P, LossParams LP,
-                                                        float* __restrict__ out,
-                                                        float* __restrict__ grad) {
-  const float* targets = LP.targets;
-  using L = WaveLds<INTERP, kWave * NG>;
-  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
-  const int lane = threadIdx.x & 63;
-  const int64_t ray = wave_ray(P);
-  if (ray >= P.n_rays) return;
-  unsigned char* mem = s_mem[threadIdx.x >> 6];
-  int* list = L::list(mem);
-  Ray R;
-  setup_ray(P, ray, R);
-  const float4 tgt = targets ? reinterpret_cast<const float4*>(targets)[ray] : camera_target(P, R);
-  const int S = P.n_samples;
-  // classify all NG chunks before the first ballot (their occupancy loads overlap), then
-  // compact the kept samples in order
-  bool kp[NG];
-#pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    const int k = (c << 6) + lane + 1;
-    kp[c] = classify<INTERP>(P, R, k) & (k <= S);
-  }
-  int nk = 0;
-#pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    const uint64_t km = __ballot(kp[c]);
-    if (kp[c]) list[nk + lane_rank(km)] = (c << 6) + lane + 1;
-    nk += __popcll(km);
-  }
-  wave_lds_sync();
-  train_ray<DT, INTERP, NG>(P, LP, R, ray, tgt, list, nk, lane, L::cells(mem), L::vals(mem), out, grad);
-}
-
 // Two rays per wavefront.  Both rays are classified together (the occupancy loads of all their
 // chunks in flight at once); when each keeps at most 32 samples (most rays of sparse scenes)
 // ray 0 runs in lanes 0-31 and ray 1 in lanes 32-63 of one group: the row-wise scans and sums
 // stop before the step that joins the two halves (half_prefix_mul, half_suffix_affine,
 // half_sums), which evaluates, for each half, the same operation tree the one-ray kernel uses
-// on lanes 0-31 -- so every value is bit-identical to k_train_step's.  Otherwise the two rays
-// run one after the other through the one-ray path.
+// on lanes 0-31 -- so every value is bit-identical to the one-ray path's (and to
+// k_render_fwd's).  Otherwise the two rays run one after the other through the one-ray path
+// (train_ray).  The register budget is capped at five waves per SIMD (95 VGPRs for trilinear,
+// S <= 256), which the one-ray path alone needed; S > 256 and FP16 trilinear would spill.
 __device__ __forceinline__ float half_prefix_mul(float x) {
   x *= dpp<DPP_ROW_SHR1>(1.f, x);
   x *= dpp<DPP_ROW_SHR2>(1.f, x);
@@ -481,7 +448,7 @@ __device__ __forceinline__ void train_half_pair(const KParams& P, const LossPara
 }
 
 template <int DT, int INTERP, int NG>
-__global__ __launch_bounds__(kBlock) void k_train_pair(KParams P, LossParams LP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG <= 4 && DT != PXL_DTYPE_F16 ? 5 : 1))) void k_train_pair(KParams P, LossParams LP,
                                                         float* __restrict__ out,
                                                         float* __restrict__ grad) {
   // per wave: [ray 0's list, then the scatter staging area | ray 1's list]
@@ -504,20 +471,16 @@ __global__ __launch_bounds__(kBlock) void k_train_pair(KParams P, LossParams LP,
   const float4 tgt0 = targets ? reinterpret_cast<const float4*>(targets)[ray0] : camera_target(P, R0);
   const float4 tgt1 = !has1 ? tgt0
                     : targets ? reinterpret_cast<const float4*>(targets)[ray0 + 1] : camera_target(P, R1);
-  const int S = P.n_samples;
-  bool kp0[NG], kp1[NG];
-#pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    const int k = (c << 6) + lane + 1;
-    kp0[c] = classify<INTERP>(P, R0, k) & (k <= S);
-    kp1[c] = classify<INTERP>(P, R1, k) & (k <= S) & has1;
-  }
+  const Ray RR[2] = {R0, R1};
+  bool kp[2][NG];
+  classify_chunks<INTERP, 2, NG>(P, RR, lane, kp);
   int n0 = 0, n1 = 0;
 #pragma unroll
   for (int c = 0; c < NG; ++c) {
-    const uint64_t km0 = __ballot(kp0[c]), km1 = __ballot(kp1[c]);
-    if (kp0[c]) list0[n0 + lane_rank(km0)] = (c << 6) + lane + 1;
-    if (kp1[c]) list1[n1 + lane_rank(km1)] = (c << 6) + lane + 1;
+    const bool k1 = kp[1][c] & has1;
+    const uint64_t km0 = __ballot(kp[0][c]), km1 = __ballot(k1);
+    if (kp[0][c]) list0[n0 + lane_rank(km0)] = (c << 6) + lane + 1;
+    if (k1) list1[n1 + lane_rank(km1)] = (c << 6) + lane + 1;
     n0 += __popcll(km0);
     n1 += __popcll(km1);
   }
@@ -548,69 +511,76 @@ __global__ void k_camera_rays(KParams P, float* __restrict__ o, float* __restric
   if (tgt) reinterpret_cast<float4*>(tgt)[r] = camera_target(P, R);
 }
 
+// Dilated occupancy: out bit of cell (x, y, z) = OR of the in bits of (x+a, y+b, z+c), a, b, c in
+// {0, 1}, wrapped -- the cells a trilinear sample with floor cell (x, y, z) reads.  One thread
+// per output word (32 cells of one tile); padding cells of partial tiles stay 0.
 __global__ void k_dilate(const uint32_t* __restrict__ in, uint32_t X, uint32_t Y, uint32_t Z,
                          uint32_t* __restrict__ out, int64_t n_words) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= n_words) return;
-  const uint64_t n_cells = (uint64_t)X * Y * Z;
+  const uint32_t TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
   uint32_t word = 0u;
-  for (int b = 0; b < 32; ++b) {
-    const uint64_t L = (uint64_t)w * 32u + b;
-    if (L >= n_cells) break;
-    const uint32_t k = (uint32_t)(L % Z), j = (uint32_t)((L / Z) % Y), i = (uint32_t)(L / ((uint64_t)Y * Z));
-    uint32_t any = 0u;
-    for (int c = 0; c < 8 && !any; ++c) {
-      const uint32_t ii = (i + (c >> 2)) % X, jj = (j + ((c >> 1) & 1)) % Y, kk = (k + (c & 1)) % Z;
-      const uint32_t Lc = (ii * Y + jj) * Z + kk;
-      any = (in[Lc >> 5] >> (Lc & 31u)) & 1u;
-    }
-    word |= any << b;
+  for (uint32_t b = 0; b < 32u; ++b) {
+    uint32_t i, j, k;
+    occ_cell((uint32_t)w, b, TY, TZ, i, j, k);
+    if (i >= X || j >= Y || k >= Z) continue;
+    const uint32_t i1 = i + 1 == X ? 0u : i + 1, j1 = j + 1 == Y ? 0u : j + 1, k1 = k + 1 == Z ? 0u : k + 1;
+    const uint32_t is[2] = {i, i1}, js[2] = {j, j1}, ks[2] = {k, k1};
+    bool any = false;
+    for (int c = 0; c < 8 && !any; ++c) any = occupied_at(in, is[c >> 2], js[(c >> 1) & 1], ks[c & 1], TY, TZ);
+    word |= (uint32_t)any << b;
   }
   out[w] = word;
 }
 
-// Occupancy from the grid's own alpha: bit L set when raw alpha >= 0 at cell L (DILATE: at any
-// of the corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped -- the cells a trilinear sample with
-// floor cell L reads).  A sample whose cells all have raw alpha < 0 is exactly zero in the
-// reference's chain: clamp gives alpha 0, so its colour weight and colour gradients vanish,
-// and the clamp's gradient mask is 0 for every alpha it read.  One lane per cell, a wave ballot
-// per 64 cells (two words).  NaN alpha counts as empty.
+// Occupancy from the grid's own alpha: bit of cell (i, j, k) set when raw alpha >= 0 there
+// (DILATE: at any of the corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped -- the cells a
+// trilinear sample with floor cell (i, j, k) reads).  A sample whose cells all have raw alpha
+// < 0 is exactly zero in the reference's chain: clamp gives alpha 0, so its colour weight and
+// colour gradients vanish, and the clamp's gradient mask is 0 for every alpha it read.  One
+// wave per pair of words (the 8 x 8 cells of one x slice of a tile: lanes 0-31 give word 2s,
+// lanes 32-63 word 2s + 1), one ballot.  NaN alpha counts as empty.
 template <int DT, bool DILATE>
 __global__ void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X, uint32_t Y, uint32_t Z,
                                   uint32_t* __restrict__ out, int64_t n_words) {
-  const int64_t n_cells = (int64_t)X * Y * Z;
   const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_cells; base += stride) {
-    const int64_t L = base + lane;
+  const uint32_t TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
+  const int64_t n_pairs = n_words >> 1;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t pr = (((int64_t)blockIdx.x * blockDim.x) >> 6) + (threadIdx.x >> 6); pr < n_pairs; pr += stride) {
+    const uint32_t w0 = (uint32_t)(pr << 1);
+    uint32_t i, j, k;
+    occ_cell(w0 + (uint32_t)(lane >> 5), (uint32_t)(lane & 31), TY, TZ, i, j, k);
     bool occ = false;
-    if (L < n_cells) {
+    if (i < X && j < Y && k < Z) {
       if constexpr (DILATE) {
-        const uint32_t k = (uint32_t)(L % Z), j = (uint32_t)((L / Z) % Y), i = (uint32_t)(L / ((int64_t)Y * Z));
         const uint32_t i1 = i + 1 == X ? 0u : i + 1, j1 = j + 1 == Y ? 0u : j + 1, k1 = k + 1 == Z ? 0u : k + 1;
         const uint32_t is[2] = {i, i1}, js[2] = {j, j1}, ks[2] = {k, k1};
 #pragma unroll
         for (int c = 0; c < 8; ++c)
           occ |= load_cell<DT>(grid, (is[c >> 2] * Y + js[(c >> 1) & 1]) * Z + ks[c & 1]).w >= 0.f;
       } else {
-        occ = load_cell<DT>(grid, (uint32_t)L).w >= 0.f;
+        occ = load_cell<DT>(grid, (i * Y + j) * Z + k).w >= 0.f;
       }
     }
     const uint64_t m = __ballot(occ);
-    const int64_t w = base >> 5;  // base is a multiple of 64
-    if (lane == 0) out[w] = (uint32_t)m;
-    if (lane == 32 && w + 1 < n_words) out[w + 1] = (uint32_t)(m >> 32);
+    if (lane == 0) out[w0] = (uint32_t)m;
+    if (lane == 32) out[w0 + 1] = (uint32_t)(m >> 32);
   }
 }
 
-__global__ void k_pack_occupancy(const uint8_t* __restrict__ occ, int64_t n_cells,
+// Bytes (X,Y,Z), nonzero = occupied, to occupancy tiles: one thread per output word.
+__global__ void k_pack_occupancy(const uint8_t* __restrict__ occ, uint32_t X, uint32_t Y, uint32_t Z,
                                  uint32_t* __restrict__ out, int64_t n_words) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= n_words) return;
+  const uint32_t TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
   uint32_t word = 0u;
-  const int64_t base = w * 32;
-  const int nb = (int)min((int64_t)32, n_cells - base);
-  for (int b = 0; b < nb; ++b) word |= (uint32_t)(occ[base + b] != 0) << b;
+  for (uint32_t b = 0; b < 32u; ++b) {
+    uint32_t i, j, k;
+    occ_cell((uint32_t)w, b, TY, TZ, i, j, k);
+    if (i < X && j < Y && k < Z) word |= (uint32_t)(occ[((size_t)i * Y + j) * Z + k] != 0) << b;
+  }
   out[w] = word;
 }
 
@@ -689,15 +659,18 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
     const uint32_t nz = min((uint32_t)kReduceCells, (uint32_t)Z - z0);
     const uint32_t xm = x == 0 ? (uint32_t)X - 1u : x - 1u;
     // source row r (0..NY) is y0 - 1 + r (r = 0 wraps at y0 = 0); rows past Y - 1 are unused
-    uint32_t base[2][NR];
+    uint32_t base[2][NR], yrow[NR];
     bool rv[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const uint32_t yr = r == 0 ? (y0 == 0 ? (uint32_t)Y - 1u : y0 - 1u) : y0 - 1u + (uint32_t)r;
       rv[r] = r == 0 || yr < (uint32_t)Y;
+      yrow[r] = yr;
       base[0][r] = (xm * (uint32_t)Y + yr) * (uint32_t)Z;
       base[1][r] = (x * (uint32_t)Y + yr) * (uint32_t)Z;
     }
+    const uint32_t xrow[2] = {xm, x};
+    const uint32_t TY = ((uint32_t)Y + 7u) >> 3, TZ = ((uint32_t)Z + 7u) >> 3;
     const uint32_t z = z0 + (uint32_t)tid;
     const bool active = (uint32_t)tid < nz;
     const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
@@ -707,8 +680,9 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
     for (int bx = 0; bx < 2; ++bx)
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        if (active && rv[r] && occupied(occ, base[bx][r] + z)) own |= 1u << (bx * NR + r);
-        if (lane == 0 && active && rv[r] && occupied(occ, base[bx][r] + zp)) own_p |= 1u << (bx * NR + r);
+        if (active && rv[r] && occupied_at(occ, xrow[bx], yrow[r], z, TY, TZ)) own |= 1u << (bx * NR + r);
+        if (lane == 0 && active && rv[r] && occupied_at(occ, xrow[bx], yrow[r], zp, TY, TZ))
+          own_p |= 1u << (bx * NR + r);
       }
     // entry (bx, r, z) may hold sums; lanes past the row's end (z >= Z) touch nothing
     const uint32_t written = active ? own | shift_up_u32(own, own_p) : 0u;
@@ -760,14 +734,16 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
 // per 64-byte entry, so each store instruction covers sixteen whole consecutive entries.
 __global__ __launch_bounds__(256) void k_blocks_zero(float4* __restrict__ blocks,
                                                      const uint32_t* __restrict__ occ,
-                                                     int Z, int64_t n_cells) {
+                                                     int Y, int Z, int64_t n_cells) {
   const int64_t n = n_cells * 4;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const uint32_t TY = ((uint32_t)Y + 7u) >> 3, TZ = ((uint32_t)Z + 7u) >> 3;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const uint32_t e = (uint32_t)(t >> 2);
-    const uint32_t ez = e % (uint32_t)Z;
-    const uint32_t ep = ez == 0 ? e + (uint32_t)Z - 1u : e - 1u;
-    if (occupied(occ, e) || occupied(occ, ep)) blocks[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t ez = e % (uint32_t)Z, ey = (e / (uint32_t)Z) % (uint32_t)Y, ex = e / ((uint32_t)Z * (uint32_t)Y);
+    const uint32_t zp = ez == 0 ? (uint32_t)Z - 1u : ez - 1u;
+    if (occupied_at(occ, ex, ey, ez, TY, TZ) || occupied_at(occ, ex, ey, zp, TY, TZ))
+      blocks[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -813,13 +789,28 @@ int check_dims(int64_t X, int64_t Y, int64_t Z) {
   return PXL_OK;
 }
 
+// Words of the occupancy tiles of an (X,Y,Z) grid: 16 per 8x8x8 tile (dims rounded up to 8).
+int64_t occupancy_words(int64_t X, int64_t Y, int64_t Z) {
+  return ((X + 7) / 8) * ((Y + 7) / 8) * ((Z + 7) / 8) * 16;
+}
+
+// The kernels index occupancy words in 32 bits.
+int check_occupancy_words(int64_t X, int64_t Y, int64_t Z) {
+  if (occupancy_words(X, Y, Z) > int64_t(0xffffffff))
+    return fail(PXL_E_SIZE, "occupancy tiles of a (%lld, %lld, %lld) grid exceed 2^32 words", (long long)X,
+                (long long)Y, (long long)Z);
+  return PXL_OK;
+}
+
 int check_grid(const pxl_grid* g) {
   if (!g) return fail(PXL_E_INVALID, "grid descriptor is NULL");
   if (!g->data) return fail(PXL_E_INVALID, "grid data is NULL");
   if (g->channels != 4) return fail(PXL_E_DTYPE, "grid must have 4 channels, got %d", g->channels);
   if (g->dtype != PXL_DTYPE_F32 && g->dtype != PXL_DTYPE_BF16 && g->dtype != PXL_DTYPE_F16)
     return fail(PXL_E_DTYPE, "unsupported grid dtype %d", g->dtype);
-  return check_dims(g->dims[0], g->dims[1], g->dims[2]);
+  int rc;
+  if ((rc = check_dims(g->dims[0], g->dims[1], g->dims[2]))) return rc;
+  return g->occupancy ? check_occupancy_words(g->dims[0], g->dims[1], g->dims[2]) : PXL_OK;
 }
 
 }  // namespace pxl_host
@@ -832,6 +823,8 @@ using pxl_host::hip_fail;
 using pxl_host::kMaxCells;
 using pxl_host::kMaxBlockCells;
 using pxl_host::check_dims;
+using pxl_host::check_occupancy_words;
+using pxl_host::occupancy_words;
 using pxl_host::launched;
 using pxl_host::stride_blocks;
 
@@ -881,6 +874,8 @@ pxl::KParams make_params(const pxl_grid* g, const pxl_march* m, const pxl_rays* 
     P.fX = (float)g->dims[0];
     P.fY = (float)g->dims[1];
     P.fZ = (float)g->dims[2];
+    P.occ_ty = (uint32_t)((g->dims[1] + 7) / 8);
+    P.occ_tz = (uint32_t)((g->dims[2] + 7) / 8);
   }
   if (m) {
     P.n_samples = m->n_samples;
@@ -1085,7 +1080,8 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
   if ((rc = launched("pxl_blocks_reduce"))) return rc;
   if (!separate) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_blocks_zero, dim3(stride_blocks(n * 4, 256)), dim3(256), 0, s,
-                     reinterpret_cast<float4*>(blocks), grid->occupancy, (int)grid->dims[2], n);
+                     reinterpret_cast<float4*>(blocks), grid->occupancy, (int)grid->dims[1],
+                     (int)grid->dims[2], n);
   return launched("pxl_blocks_reduce");
 }
 
@@ -1109,7 +1105,9 @@ int pxl_occupancy_dilate(const uint32_t* occ_in, int64_t X, int64_t Y, int64_t Z
   if (occ_in == occ_out) return fail(PXL_E_INVALID, "occ_out must not alias occ_in");
   if (X <= 0 || Y <= 0 || Z <= 0) return fail(PXL_E_INVALID, "dims must be positive");
   if (X * Y > kMaxCells || X * Y * Z > kMaxCells) return fail(PXL_E_SIZE, "grid has more than 2^31-1 cells");
-  const int64_t words = (X * Y * Z + 31) / 32;
+  int rc;
+  if ((rc = check_occupancy_words(X, Y, Z))) return rc;
+  const int64_t words = occupancy_words(X, Y, Z);
   hipLaunchKernelGGL(pxl::k_dilate, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, occ_in, (uint32_t)X, (uint32_t)Y, (uint32_t)Z, occ_out, words);
   const hipError_t e = hipGetLastError();
@@ -1120,9 +1118,9 @@ int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, vo
   int rc;
   if ((rc = check_grid(grid))) return rc;
   if (!bits) return fail(PXL_E_INVALID, "bits is NULL");
-  const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
-  const int64_t words = (n + 31) / 32;
-  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, int64_t(1) << 20);
+  if ((rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  const int64_t words = occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]);
+  const unsigned nb = (unsigned)std::min<int64_t>((words / 2 + 3) / 4, int64_t(1) << 20);  // wave per word pair
   hipStream_t s = (hipStream_t)stream;
   const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
 #define PXL_ALPHA_OCC(DTV)                                                                           \
@@ -1143,16 +1141,20 @@ int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, vo
   return launched("pxl_alpha_occupancy");
 }
 
-int pxl_pack_occupancy(const uint8_t* occ, int64_t n_cells, uint32_t* out, void* stream) {
+int pxl_pack_occupancy(const uint8_t* occ, int64_t X, int64_t Y, int64_t Z, uint32_t* out, void* stream) {
+  int rc;
   if (!occ || !out) return fail(PXL_E_INVALID, "occ and out are required");
-  if (n_cells < 0) return fail(PXL_E_INVALID, "n_cells is negative");
-  if (n_cells > kMaxCells) return fail(PXL_E_SIZE, "more than 2^31-1 cells");
-  if (n_cells == 0) return PXL_OK;
-  const int64_t words = (n_cells + 31) / 32;
+  if ((rc = check_dims(X, Y, Z)) || (rc = check_occupancy_words(X, Y, Z))) return rc;
+  const int64_t words = occupancy_words(X, Y, Z);
   hipLaunchKernelGGL(pxl::k_pack_occupancy, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, occ, n_cells, out, words);
+                     (hipStream_t)stream, occ, (uint32_t)X, (uint32_t)Y, (uint32_t)Z, out, words);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_pack_occupancy");
+}
+
+int64_t pxl_occupancy_words(int64_t X, int64_t Y, int64_t Z) {
+  if (X <= 0 || Y <= 0 || Z <= 0 || X > kMaxCells || Y > kMaxCells || Z > kMaxCells) return -1;
+  return occupancy_words(X, Y, Z);
 }
 
 int pxl_fill_f32(float* dst, int64_t n, float value, void* stream) {

@@ -80,7 +80,7 @@ def _grid_struct(grid, occupancy=None):
     if occupancy is not None:
         if occupancy.dtype != torch.int32 or not occupancy.is_cuda or not occupancy.is_contiguous():
             raise ValueError("occupancy must be a contiguous int32 device tensor of packed bits")
-        need = (grid.shape[0] * grid.shape[1] * grid.shape[2] + 31) // 32
+        need = occupancy_words(grid.shape[:3])
         if occupancy.numel() < need:
             raise ValueError(f"occupancy needs {need} words, got {occupancy.numel()}")
         g.occupancy = occupancy.data_ptr()
@@ -355,9 +355,20 @@ def fill_(t, value=0.0):
     return t
 
 
+def occupancy_words(dims):
+    """int32 words of the occupancy bits of an (X,Y,Z) grid: 16 per 8x8x8 tile (pxl.h)."""
+    X, Y, Z = (int(v) for v in dims)
+    n = int(_lib.lib().pxl_occupancy_words(X, Y, Z))
+    if n < 0:
+        raise ValueError(f"invalid grid dims {(X, Y, Z)}")
+    return n
+
+
 def occupancy_dilate(occ_bits, dims):
     """Cell occupancy for trilinear skipping from point occupancy bits (see pxl.h)."""
     X, Y, Z = dims
+    if occ_bits.numel() < occupancy_words(dims):
+        raise ValueError(f"occupancy needs {occupancy_words(dims)} words, got {occ_bits.numel()}")
     out = torch.empty_like(occ_bits)
     _lib.check(_lib.lib().pxl_occupancy_dilate(_lib.ptr(occ_bits), X, Y, Z, _lib.ptr(out),
                                                _lib.stream_handle(occ_bits.device)))
@@ -370,7 +381,7 @@ def alpha_occupancy(grid, interp="nearest"):
     changes no finite result of the render and training kernels: their clamped alpha is 0 and
     the clamp's gradient mask is 0."""
     X, Y, Z = (int(v) for v in grid.shape[:3])
-    bits = torch.empty((X * Y * Z + 31) // 32, dtype=torch.int32, device=grid.device)
+    bits = torch.empty(occupancy_words((X, Y, Z)), dtype=torch.int32, device=grid.device)
     g = _grid_struct(grid, None)
     _lib.check(_lib.lib().pxl_alpha_occupancy(ctypes.byref(g), 1 if interp == "trilinear" else 0, _lib.ptr(bits),
                                               _lib.stream_handle(grid.device)))
@@ -378,14 +389,15 @@ def alpha_occupancy(grid, interp="nearest"):
 
 
 def pack_occupancy(occ):
-    """(X,Y,Z) bool/uint8 device tensor -> packed int32 bit words for the kernels' occupancy
-    argument (bit (L & 31) of word L >> 5, L the linear cell index)."""
+    """(X,Y,Z) bool/uint8 device tensor -> int32 bit words for the kernels' occupancy argument
+    (8x8x8 tiles of 16 words; cell (x,y,z) is bit (y&3)*8 + (z&7) of word (x&7)*2 + ((y>>2)&1)
+    of tile ((x>>3)*TY + (y>>3))*TZ + (z>>3), TY = ceil(Y/8), TZ = ceil(Z/8))."""
     if not occ.is_cuda or occ.dim() != 3:
         raise ValueError("occupancy must be an (X,Y,Z) device tensor")
     src = occ.to(torch.uint8).contiguous()
-    n = src.numel()
-    out = torch.empty((n + 31) // 32, dtype=torch.int32, device=occ.device)
-    _lib.check(_lib.lib().pxl_pack_occupancy(_lib.ptr(src), n, _lib.ptr(out),
+    X, Y, Z = (int(v) for v in src.shape)
+    out = torch.empty(occupancy_words((X, Y, Z)), dtype=torch.int32, device=occ.device)
+    _lib.check(_lib.lib().pxl_pack_occupancy(_lib.ptr(src), X, Y, Z, _lib.ptr(out),
                                              _lib.stream_handle(occ.device)))
     return out
 

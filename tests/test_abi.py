@@ -150,6 +150,22 @@ def test_validation_errors_before_any_hip_call():
     assert L.pxl_fill_f32(None, 4, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_fill_f32(ctypes.c_void_p(0x1004), 8, ctypes.c_float(0.0), None) == _lib.PXL_E_INVALID
     assert L.pxl_occupancy_dilate(ctypes.c_void_p(0x10), 4, 4, 4, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
+    # occupancy tiles: 16 words per 8x8x8 tile, dims rounded up to 8
+    assert L.pxl_occupancy_words(5, 6, 7) == 16
+    assert L.pxl_occupancy_words(9, 17, 10) == 2 * 3 * 2 * 16
+    assert L.pxl_occupancy_words(256, 256, 256) == 256 ** 3 // 32
+    assert L.pxl_occupancy_words(0, 4, 4) == -1
+    assert L.pxl_pack_occupancy(None, 4, 4, 4, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
+    assert L.pxl_pack_occupancy(ctypes.c_void_p(0x10), 4, 0, 4, ctypes.c_void_p(0x20), None) == _lib.PXL_E_INVALID
+    # 1 x 1 x (2^31 - 1) pads to 2^28 tiles = 2^32 words, past the kernels' 32-bit word index
+    assert L.pxl_occupancy_dilate(ctypes.c_void_p(0x10), 1, 1, 2 ** 31 - 1, ctypes.c_void_p(0x20), None) == _lib.PXL_E_SIZE
+    g4, m4, r4 = _valid()
+    g4.dims[0] = g4.dims[1] = 1
+    g4.dims[2] = 2 ** 31 - 1
+    g4.occupancy = ctypes.c_void_p(0x40)
+    assert L.pxl_render_fwd(ctypes.byref(g4), ctypes.byref(m4), ctypes.byref(r4), ctypes.c_void_p(0x50), None,
+                            None) == _lib.PXL_E_SIZE
+    assert "occupancy" in _lib.last_error()
     assert L.pxl_aux_floats(256) == 10
     assert L.pxl_aux_floats(0) == 2
     assert L.pxl_aux_floats(65) == 6

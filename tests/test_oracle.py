@@ -283,13 +283,30 @@ def test_trilinear_boundary_quirks():
     assert torch.allclose(vals[0], want)
 
 
-def test_occupancy_bits_roundtrip():
-    g = torch.Generator().manual_seed(1)
-    occ = torch.rand(5, 6, 7, generator=g) < 0.3
+@pytest.mark.parametrize("dims", [(5, 6, 7), (9, 17, 10), (8, 8, 16)])
+def test_occupancy_bits_roundtrip(dims):
+    """The vectorised packing against the tile layout cell by cell (8x8x8 tiles of 16 words;
+    cell (x,y,z): word tile*16 + (x&7)*2 + ((y>>2)&1), bit (y&3)*8 + (z&7)); every bit of the
+    words is either a cell's or padding, and padding is 0."""
+    g = torch.Generator().manual_seed(sum(dims))
+    occ = torch.rand(*dims, generator=g) < 0.3
     words = plain.occupancy_to_bits(occ).numpy().view(np.uint32)
-    flat = occ.reshape(-1).numpy()
-    for L in range(flat.size):
-        assert bool((words[L >> 5] >> (L & 31)) & 1) == bool(flat[L])
+    X, Y, Z = dims
+    TY, TZ = (Y + 7) // 8, (Z + 7) // 8
+    assert words.size == ((X + 7) // 8) * TY * TZ * 16
+    seen = np.zeros(words.size * 32, dtype=bool)
+    o = occ.numpy()
+    for x in range(X):
+        for y in range(Y):
+            for z in range(Z):
+                t = ((x // 8) * TY + y // 8) * TZ + z // 8
+                w = t * 16 + (x % 8) * 2 + (y % 8) // 4
+                b = (y % 4) * 8 + z % 8
+                assert not seen[w * 32 + b], "two cells share a bit"
+                seen[w * 32 + b] = True
+                assert bool((words[w] >> b) & 1) == bool(o[x, y, z])
+    allbits = np.unpackbits(words.view(np.uint8), bitorder="little").astype(bool)
+    assert not bool((allbits & ~seen).any()), "a padding bit is set"
 
 
 # --------------------------------------------------------------------------- C oracle vs plain
