@@ -306,6 +306,18 @@ int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_av
                   float* grad_abs, int64_t n, double lr, double beta1, double beta2, double eps,
                   int64_t step, void* half_out, int32_t half_dtype, void* stream);
 
+/* pxl_tv_bwd followed by pxl_adam_step in one pass over an (X,Y,Z,4) grid (scripts/train.py:
+ * 163-168 and :180-184 for a step with the TV term): the total gradient grad + d(tv_weight *
+ * sqrt(tv_sum_sq))/dparam is formed in registers, exactly as pxl_tv_bwd would add it into grad
+ * (grad itself is not written), and drives the Adam update, the |grad| accumulation and the
+ * half copy; the result equals the two separate calls bit for bit.  The updated parameter is
+ * written to param_out, which must be a different buffer from param_in: the TV stencil reads
+ * each cell's six neighbours from param_in.  Arrays 16-byte aligned (half_out 8-byte). */
+int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad, float* exp_avg,
+                     float* exp_avg_sq, float* grad_abs, int64_t X, int64_t Y, int64_t Z,
+                     const double* tv_sum_sq, float tv_weight, double lr, double beta1, double beta2,
+                     double eps, int64_t step, void* half_out, int32_t half_dtype, void* stream);
+
 /* Host-side traffic accounting (no counterpart in the reference; ALL POINTERS ARE HOST
  * POINTERS, including grid->occupancy, and grid->data is not read): counts[r] = number of
  * distinct grid cells the processed samples of ray r read -- nearest cell, or the union of

@@ -594,6 +594,69 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   }
 }
 
+// Adam on grad + the TV gradient of the parameter, in one pass: the TV term of k_tv_bwd is formed
+// in registers from p_in and its six neighbours and added to the gradient exactly as k_tv_bwd
+// adds it (same operations, same order), then the Adam update of k_adam follows; |grad| and the
+// half copy see the total gradient and the new parameter.  The update goes to p_out, not p_in:
+// a neighbour updated in place would change the stencil of the cells read after it.
+__global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in, float4* __restrict__ p_out,
+                                                 const float4* __restrict__ g, float4* __restrict__ m,
+                                                 float4* __restrict__ v, float4* __restrict__ gabs, Dims d,
+                                                 const double* __restrict__ sum, float weight, float w1,
+                                                 float beta2, float w2, float bc2_sqrt, float eps,
+                                                 float neg_step, void* __restrict__ half_out, int half_dtype) {
+  const float scale = weight / (float)sqrt(*sum);
+  const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
+  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
+    uint32_t x, y, z;
+    cell_xyz((uint32_t)L, Y, Z, x, y, z);
+    float4 P = p_in[L];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t nb[6] = {y + 1 < Y ? L + Z : -1,  y > 0 ? L - Z : -1,
+                           z + 1 < Z ? L + 1 : -1,  z > 0 ? L - 1 : -1,
+                           x + 1 < X ? L + YZ : -1, x > 0 ? L - YZ : -1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if (nb[q] < 0) continue;
+      const float4 u = p_in[nb[q]];
+      acc.x += P.x - u.x;
+      acc.y += P.y - u.y;
+      acc.z += P.z - u.z;
+      acc.w += P.w - u.w;
+    }
+    float4 G = g[L];
+    G.x += scale * acc.x;
+    G.y += scale * acc.y;
+    G.z += scale * acc.z;
+    G.w += scale * acc.w;
+    float4 M = m[L], V = v[L];
+    adam_elem(P.x, G.x, M.x, V.x, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.y, G.y, M.y, V.y, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.z, G.z, M.z, V.z, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(P.w, G.w, M.w, V.w, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    p_out[L] = P;
+    m[L] = M;
+    v[L] = V;
+    if (gabs) {
+      float4 A = gabs[L];
+      A = make_float4(A.x + fabsf(G.x), A.y + fabsf(G.y), A.z + fabsf(G.z), A.w + fabsf(G.w));
+      gabs[L] = A;
+    }
+    if (half_out) {
+      if (half_dtype == PXL_DTYPE_BF16) {
+        const uint32_t lo = to_bf16_rne(P.x) | ((uint32_t)to_bf16_rne(P.y) << 16);
+        const uint32_t hi = to_bf16_rne(P.z) | ((uint32_t)to_bf16_rne(P.w) << 16);
+        reinterpret_cast<uint2*>(half_out)[L] = make_uint2(lo, hi);
+      } else {
+        store_half(half_out, half_dtype, 4 * L + 0, P.x);
+        store_half(half_out, half_dtype, 4 * L + 1, P.y);
+        store_half(half_out, half_dtype, 4 * L + 2, P.z);
+        store_half(half_out, half_dtype, 4 * L + 3, P.w);
+      }
+    }
+  }
+}
+
 }  // namespace
 }  // namespace pxl
 
@@ -883,6 +946,36 @@ int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double*
                      (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq, weight,
                      reinterpret_cast<float4*>(grad_grid));
   return launched("pxl_tv_bwd");
+}
+
+int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad, float* exp_avg,
+                     float* exp_avg_sq, float* grad_abs, int64_t X, int64_t Y, int64_t Z,
+                     const double* tv_sum_sq, float tv_weight, double lr, double beta1, double beta2,
+                     double eps, int64_t step, void* half_out, int32_t half_dtype, void* stream) {
+  int rc;
+  auto al16 = [](const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+  if ((rc = pxl_host::check_dims(X, Y, Z)) ||
+      (rc = need(param_in && param_out && grad && exp_avg && exp_avg_sq && tv_sum_sq,
+                 "param_in, param_out, grad, moments and tv_sum_sq required")) ||
+      (rc = need(param_in != param_out, "param_out must not be param_in (the TV stencil reads neighbours)")) ||
+      (rc = need(step >= 1, "step counts from 1")) ||
+      (rc = need(!half_out || half_dtype == PXL_DTYPE_BF16 || half_dtype == PXL_DTYPE_F16,
+                 "half_dtype must be PXL_DTYPE_BF16 or PXL_DTYPE_F16")) ||
+      (rc = need(al16(param_in) && al16(param_out) && al16(grad) && al16(exp_avg) && al16(exp_avg_sq) &&
+                     al16(grad_abs) && (half_out == nullptr || (reinterpret_cast<uintptr_t>(half_out) & 7u) == 0),
+                 "float arrays must be 16-byte aligned (half_out 8-byte)")))
+    return rc;
+  const double bc1 = 1.0 - std::pow(beta1, (double)step);
+  const double bc2 = 1.0 - std::pow(beta2, (double)step);
+  const float neg_step = (float)(-(lr / bc1));
+  const float bc2_sqrt = (float)std::pow(bc2, 0.5);
+  hipLaunchKernelGGL(pxl::k_adam_tv, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(param_in), reinterpret_cast<float4*>(param_out),
+                     reinterpret_cast<const float4*>(grad), reinterpret_cast<float4*>(exp_avg),
+                     reinterpret_cast<float4*>(exp_avg_sq), reinterpret_cast<float4*>(grad_abs), Dims{X, Y, Z},
+                     tv_sum_sq, tv_weight, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2), bc2_sqrt,
+                     (float)eps, neg_step, half_out, half_dtype);
+  return launched("pxl_adam_tv_step");
 }
 
 int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,

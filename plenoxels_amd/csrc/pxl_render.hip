@@ -537,20 +537,27 @@ __global__ void k_dilate(const uint32_t* __restrict__ in, uint32_t X, uint32_t Y
 // (DILATE: at any of the corners (i+a, j+b, k+c), a,b,c in {0,1}, wrapped -- the cells a
 // trilinear sample with floor cell (i, j, k) reads).  A sample whose cells all have raw alpha
 // < 0 is exactly zero in the reference's chain: clamp gives alpha 0, so its colour weight and
-// colour gradients vanish, and the clamp's gradient mask is 0 for every alpha it read.  One
-// wave per pair of words (the 8 x 8 cells of one x slice of a tile: lanes 0-31 give word 2s,
-// lanes 32-63 word 2s + 1), one ballot.  NaN alpha counts as empty.
+// colour gradients vanish, and the clamp's gradient mask is 0 for every alpha it read.  NaN
+// alpha counts as empty.
+// A workgroup covers one unit: x, a y quad (y = 4q .. 4q+3, one wave each) and 64 consecutive z
+// (the lanes, so each wave reads whole runs of cells).  Each wave's ballot holds one byte-row
+// (y & 3) of the 8 tile words along z; four lanes assemble the words from the four ballots in
+// LDS.  Units cover the padded dims (multiples of 8), so every word, padding included, is
+// written.
+constexpr int kAlphaOccThreads = 256;
+
 template <int DT, bool DILATE>
-__global__ void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X, uint32_t Y, uint32_t Z,
-                                  uint32_t* __restrict__ out, int64_t n_words) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
-  const int64_t n_pairs = n_words >> 1;
-  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t pr = (((int64_t)blockIdx.x * blockDim.x) >> 6) + (threadIdx.x >> 6); pr < n_pairs; pr += stride) {
-    const uint32_t w0 = (uint32_t)(pr << 1);
-    uint32_t i, j, k;
-    occ_cell(w0 + (uint32_t)(lane >> 5), (uint32_t)(lane & 31), TY, TZ, i, j, k);
+__global__ __launch_bounds__(kAlphaOccThreads) void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X,
+                                                                      uint32_t Y, uint32_t Z,
+                                                                      uint32_t* __restrict__ out) {
+  __shared__ uint64_t s_m[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t TX = (X + 7u) >> 3, TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
+  const uint32_t ZB = (TZ + 7u) >> 3, YQ = TY * 2u;
+  const uint64_t n_units = (uint64_t)(TX * 8u) * YQ * ZB;
+  for (uint64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const uint32_t zb = (uint32_t)(u % ZB), yq = (uint32_t)((u / ZB) % YQ), i = (uint32_t)(u / ((uint64_t)ZB * YQ));
+    const uint32_t j = yq * 4u + (uint32_t)wave, k = zb * 64u + (uint32_t)lane;
     bool occ = false;
     if (i < X && j < Y && k < Z) {
       if constexpr (DILATE) {
@@ -564,8 +571,19 @@ __global__ void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X, uin
       }
     }
     const uint64_t m = __ballot(occ);
-    if (lane == 0) out[w0] = (uint32_t)m;
-    if (lane == 32) out[w0 + 1] = (uint32_t)(m >> 32);
+    if (lane == 0) s_m[wave] = m;
+    __syncthreads();
+    if (threadIdx.x < 8u) {  // tile word of z tile 8 zb + t
+      const uint32_t t = threadIdx.x, tz = zb * 8u + t;
+      if (tz < TZ) {
+        uint32_t word = 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) word |= (uint32_t)((s_m[w] >> (8u * t)) & 0xffu) << (8 * w);
+        const uint32_t tile = ((i >> 3) * TY + (yq >> 1)) * TZ + tz;
+        out[(size_t)tile * 16u + (i & 7u) * 2u + (yq & 1u)] = word;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1119,17 +1137,18 @@ int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, vo
   if ((rc = check_grid(grid))) return rc;
   if (!bits) return fail(PXL_E_INVALID, "bits is NULL");
   if ((rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
-  const int64_t words = occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]);
-  const unsigned nb = (unsigned)std::min<int64_t>((words / 2 + 3) / 4, int64_t(1) << 20);  // wave per word pair
+  const int64_t units = ((grid->dims[0] + 7) / 8 * 8) * ((grid->dims[1] + 7) / 8 * 2) *
+                        (((grid->dims[2] + 7) / 8 + 7) / 8);
+  const unsigned nb = (unsigned)std::min<int64_t>(units, int64_t(1) << 20);  // workgroup per unit
   hipStream_t s = (hipStream_t)stream;
   const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
 #define PXL_ALPHA_OCC(DTV)                                                                           \
   if (dilate)                                                                                        \
-    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, true>), dim3(nb), dim3(256), 0, s, grid->data, X, Y, Z, \
-                       bits, words);                                                                 \
+    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, true>), dim3(nb), dim3(pxl::kAlphaOccThreads), 0, s,     \
+                       grid->data, X, Y, Z, bits);                                                   \
   else                                                                                               \
-    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, false>), dim3(nb), dim3(256), 0, s, grid->data, X, Y, Z, \
-                       bits, words);
+    hipLaunchKernelGGL((pxl::k_alpha_occupancy<DTV, false>), dim3(nb), dim3(pxl::kAlphaOccThreads), 0, s,    \
+                       grid->data, X, Y, Z, bits);
   if (grid->dtype == PXL_DTYPE_F32) {
     PXL_ALPHA_OCC(PXL_DTYPE_F32)
   } else if (grid->dtype == PXL_DTYPE_BF16) {

@@ -21,7 +21,7 @@ import torch
 from . import _lib
 from .checkpoint import save_grid
 from .grid_functions import average_pool3d_grid, pool_backward_
-from .optim import adam_step_, tv_backward_, tv_sumsq
+from .optim import adam_step_, adam_tv_step_, tv_backward_, tv_sumsq
 from .render import CameraRays, CornerBlockPair, MarchConfig, alpha_occupancy, beta_term_zero, fill_, train_step
 
 RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
@@ -102,6 +102,7 @@ class Trainer:
         self.sq = torch.empty(n, device=self.dev)
         self.bsum = torch.empty(n, device=self.dev)
         self.tvs = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self._grid_next = None  # second FP32 grid of the fused TV + Adam update (allocated on first use)
         if self.even_spread:
             m = int(round(math.sqrt(self.R)))
             ls = torch.linspace(0, end=1, steps=m, device=self.dev)
@@ -153,11 +154,15 @@ class Trainer:
         closs = self.sq.sum() / (4 * n_rays)
         out = {"closs": closs}
         loss = closs
+        # full-grid steps with the TV term fold its backward into the Adam pass (pxl_adam_tv_step,
+        # the same bits), which writes the updated grid into a second buffer
+        fused_tv = self.tv > 0 and k < 19 and not pooled
         if self.tv > 0 and k < 19:
             tv_grid = grid if grid.dtype == torch.float32 else self.grid
             self.tvs.zero_()
             tv_sumsq(tv_grid, out=self.tvs)
-            tv_backward_(tv_grid, self.tvs, self.tv, grad)
+            if not fused_tv:
+                tv_backward_(tv_grid, self.tvs, self.tv, grad)
             tvl = (self.tv * torch.sqrt(self.tvs)).float().reshape(())
             out["tvloss"] = tvl
             loss = loss + tvl
@@ -170,8 +175,16 @@ class Trainer:
             fill_(self.grad, 0.0)
             pool_backward_(grad, (X, Y, Z), k, max(1, k // 4), self.grad)
         self.step_count += 1
-        adam_step_(self.grid, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count, lr=self.lr,
-                   grad_abs=self.grad_abs, half_out=self.render_copy)
+        if fused_tv:
+            if self._grid_next is None:
+                self._grid_next = torch.empty_like(self.grid)
+            adam_tv_step_(self.grid, self._grid_next, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count,
+                          tv_sumsq=self.tvs, tv_weight=self.tv, lr=self.lr, grad_abs=self.grad_abs,
+                          half_out=self.render_copy)
+            self.grid, self._grid_next = self._grid_next, self.grid
+        else:
+            adam_step_(self.grid, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count, lr=self.lr,
+                       grad_abs=self.grad_abs, half_out=self.render_copy)
         out["loss"] = loss
         return out
 

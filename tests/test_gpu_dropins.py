@@ -220,6 +220,44 @@ def test_adam_matches_torch():
     assert torch.equal(half.cpu(), pd.detach().cpu().to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("half_dtype", [None, torch.bfloat16, torch.float16])
+def test_fused_tv_adam_equals_tv_backward_then_adam(half_dtype):
+    """pxl_adam_tv_step: the bits of pxl_tv_bwd into the gradient followed by pxl_adam_step,
+    over three steps on an odd-shaped grid (faces, edges and corners of the stencil), with the
+    parameter ping-ponged between two buffers; the gradient buffer is left unchanged."""
+    from plenoxels_amd.optim import adam_step_, adam_tv_step_, tv_backward_, tv_sumsq
+    say("fused TV + Adam", half_dtype)
+    g = torch.Generator().manual_seed(14)
+    dims = (9, 6, 11)
+    p0 = torch.randn(*dims, 4, generator=g).to(DEV)
+    grads = [torch.randn(*dims, 4, generator=g).to(DEV) * 10 ** (-i) for i in range(3)]
+    a_p, a_m, a_v, a_ga = p0.clone(), torch.zeros_like(p0), torch.zeros_like(p0), torch.zeros_like(p0)
+    b_p, b_alt, b_m, b_v, b_ga = p0.clone(), torch.empty_like(p0), torch.zeros_like(p0), torch.zeros_like(p0), \
+        torch.zeros_like(p0)
+    a_h = None if half_dtype is None else torch.empty_like(p0, dtype=half_dtype)
+    b_h = None if half_dtype is None else torch.empty_like(p0, dtype=half_dtype)
+    for t, gr in enumerate(grads, start=1):
+        # one TV sum for both paths (its FP64 atomics make repeated sums differ in the last bit)
+        assert torch.equal(a_p, b_p)
+        s_t = tv_sumsq(a_p)
+        # separate passes on a copy of the gradient
+        ga = gr.clone()
+        tv_backward_(a_p, s_t, 0.37, ga)
+        adam_step_(a_p, ga, a_m, a_v, t, lr=0.01, grad_abs=a_ga, half_out=a_h)
+        # fused pass
+        gb = gr.clone()
+        adam_tv_step_(b_p, b_alt, gb, b_m, b_v, t, tv_sumsq=s_t, tv_weight=0.37, lr=0.01, grad_abs=b_ga,
+                      half_out=b_h)
+        assert torch.equal(gb, gr), "the fused step must not write the gradient"
+        b_p, b_alt = b_alt, b_p
+        for x, y in ((a_p, b_p), (a_m, b_m), (a_v, b_v), (a_ga, b_ga)):
+            assert torch.equal(x, y)
+        if half_dtype is not None:
+            assert torch.equal(a_h, b_h)
+    with pytest.raises(ValueError):
+        adam_tv_step_(b_p, b_p, grads[0], b_m, b_v, 4, tv_sumsq=s_t, tv_weight=0.37, lr=0.01)
+
+
 @pytest.mark.parametrize("interp", ["nearest", "trilinear"])
 def test_train_step_beta_term(interp):
     """The beta regulariser of scripts/train.py:170-177 fused into the train step."""
