@@ -174,6 +174,12 @@ int64_t pxl_blocks_floats(const pxl_grid* grid);
 int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int32_t accumulate,
                       float* clear, void* stream);
 
+/* Zero the entries of corner blocks that a training step with grid->occupancy can write (every
+ * entry when occupancy is NULL) -- the clearing pxl_blocks_reduce does with clear == blocks,
+ * as a call of its own, e.g. on a second stream while the other buffer of a pair is in use.
+ * grid->data is not read. */
+int pxl_blocks_clear(const pxl_grid* grid, float* blocks, void* stream);
+
 /* Ray generation only (src/ray_sampling.py:195-264 on per-ray camera/uniforms):
  * origins (n,3), unit dirs (n,3), targets (n,4) (NULL to skip; needs imgs). */
 int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,

@@ -70,6 +70,7 @@ SIGNATURES = {
     "pxl_blocks_floats": (ctypes.c_int64, [_GP]),
     "pxl_blocks_reduce": (ctypes.c_int, [_GP, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pxl_blocks_clear": (ctypes.c_int, [_GP, c_void_p, c_void_p]),
     "pxl_occupancy_dilate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, c_void_p, c_void_p]),
     "pxl_pack_occupancy": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,

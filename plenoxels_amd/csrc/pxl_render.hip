@@ -1103,6 +1103,22 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
   return launched("pxl_blocks_reduce");
 }
 
+int pxl_blocks_clear(const pxl_grid* grid, float* blocks, void* stream) {
+  int rc;
+  if (!grid) return fail(PXL_E_INVALID, "grid descriptor is NULL");
+  if ((rc = check_dims(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  if (grid->occupancy && (rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  if (!blocks) return fail(PXL_E_INVALID, "blocks is NULL");
+  const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
+  if (n > kMaxBlockCells)
+    return fail(PXL_E_SIZE, "corner blocks index entry*4 + slot in 32 bits: at most %lld cells",
+                (long long)kMaxBlockCells);
+  hipLaunchKernelGGL(pxl::k_blocks_zero, dim3(stride_blocks(n * 4, 256)), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<float4*>(blocks), grid->occupancy, (int)grid->dims[1],
+                     (int)grid->dims[2], n);
+  return launched("pxl_blocks_clear");
+}
+
 int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* targets,
                     void* stream) {
   int rc;

@@ -236,16 +236,27 @@ class CornerBlockPair:
 def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clear=None):
     """grad_grid (+)= the gradient held in corner blocks (pxl_blocks_reduce).  clear: the
     blocks to zero for the next step -- default `blocks` itself (a second pass); another
-    buffer is zeroed within the same pass.  occupancy: the one the training steps used."""
+    buffer is zeroed within the same pass; False: none (clear them with blocks_clear).
+    occupancy: the one the training steps used."""
     L = _lib.lib()
     _check_blocks(grid, blocks, grad_grid)
     if clear is None:
         clear = blocks
+    elif clear is False:
+        clear = None
     elif clear is not blocks:
         _check_blocks(grid, clear, None)
     g = _grid_struct(grid, occupancy)
     _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid), 1 if accumulate else 0,
                                    _lib.ptr(clear), _lib.stream_handle(grid.device)))
+
+
+def blocks_clear(grid, blocks, occupancy=None):
+    """Zero the corner-block entries a step with `occupancy` can write (pxl_blocks_clear), on the
+    current stream."""
+    _check_blocks(grid, blocks, None)
+    g = _grid_struct(grid, occupancy)
+    _lib.check(_lib.lib().pxl_blocks_clear(ctypes.byref(g), _lib.ptr(blocks), _lib.stream_handle(grid.device)))
 
 
 def _check_blocks(grid, blocks, grad_grid):

@@ -197,6 +197,8 @@ def test_corner_block_validation_before_any_hip_call():
     assert L.pxl_train_step_blocks(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss), None,
                                    None, None) == _lib.PXL_E_INVALID
     assert L.pxl_blocks_floats(ctypes.byref(g)) == 8 * 8 * 8 * 16
+    assert L.pxl_blocks_clear(ctypes.byref(g), None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_blocks_clear(None, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
     g.dims[1] = 0
     assert L.pxl_blocks_floats(ctypes.byref(g)) == -1
     assert L.pxl_blocks_reduce(ctypes.byref(g), ctypes.c_void_p(0x10), ctypes.c_void_p(0x20), 1, None, None) == _lib.PXL_E_INVALID

@@ -165,6 +165,44 @@ def test_trainer_corner_blocks_match_direct(start, dims, progressive):
 
 
 @pytest.mark.parametrize("occupied", [False, True])
+def test_reduce_without_clear_then_blocks_clear(occupied):
+    """reduce(clear=False) leaves the blocks as they were; blocks_clear then zeroes exactly the
+    entries a step with that occupancy can write (entry (x, y, z): bit of (x, y, z) or of
+    (x, y, z - 1 wrapped)), every entry without occupancy."""
+    from plenoxels_amd import blocks_clear
+    say("reduce without clear + blocks_clear", "occupancy" if occupied else "dense")
+    grid, o, d, gmin, pd = small_scene(95, X=20, R=256)
+    tgt = torch.rand(o.shape[0], 4, generator=torch.Generator().manual_seed(96))
+    occ_dev, written = None, None
+    if occupied:
+        occ = torch.rand(grid.shape[:3], generator=torch.Generator().manual_seed(97)) < 0.35
+        cells = plain.dilate_cells(occ)
+        written = (cells | torch.roll(cells, shifts=1, dims=2)).reshape(-1)
+        occ_dev = occupancy_dilate(pack_occupancy(occ.to(DEV)), grid.shape[:3])
+    cfg = MarchConfig(120, 0.035, tuple(gmin.tolist()), pd, "trilinear")
+    gd = grid.to(DEV)
+    blocks = corner_blocks(gd)
+    train_step(gd, None, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), occupancy=occ_dev,
+               blocks=blocks)
+    before = blocks.clone()
+    g = torch.empty(grid.shape, device=DEV)
+    blocks_reduce(gd, blocks, g, occupancy=occ_dev, accumulate=False, clear=False)
+    assert torch.equal(blocks, before)
+    g_direct = torch.zeros(grid.shape, device=DEV)
+    train_step(gd, g_direct, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), occupancy=occ_dev)
+    ok, info = grad_close(g.cpu(), g_direct.cpu(), rel=1e-5)
+    assert ok, info
+    blocks.fill_(1.0)
+    blocks_clear(gd, blocks, occupancy=occ_dev)
+    b = blocks.cpu()
+    if occupied:
+        assert int(torch.count_nonzero(b[written])) == 0
+        assert bool((b[~written] == 1.0).all())
+    else:
+        assert int(torch.count_nonzero(b)) == 0
+
+
+@pytest.mark.parametrize("occupied", [False, True])
 def test_double_buffer_reduce_clears_the_other_buffer(occupied):
     """reduce(blocks=A, clear=B): the gradient equals the single-buffer reduce; B's entries that
     a step can write (all entries without occupancy) are zeroed in the same pass and A is left
