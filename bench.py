@@ -12,7 +12,7 @@ lookup, occupancy skipping, early termination at transmittance 1e-4.  Synthetic,
 
 One step = pxl_camera_rays (ray generation from (camera, u, v) and target gather) +
 pxl_train_step_blocks (sample placement, gather, compositing, MSE gradient, backward scatter
-into per-floor-cell corner blocks) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
+into per-cell corner-block entries) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
 zero-fill -- and zeroes the other of two alternating block buffers) + the loss reduction.  --scatter direct times the
 zero-fill + pxl_train_step with atomics straight into the gradient instead (nearest always
 scatters directly: one cell per sample).  Adam is not part of the step (reference

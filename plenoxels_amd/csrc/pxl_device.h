@@ -516,7 +516,8 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomic
 
 // Gradient items per sample, and the LDS staging capacity: the items of kStageSamples kept
 // samples are staged and flushed at a time.  Trilinear stages 32 samples (256 items, 5 KiB
-// per wave) so that LDS leaves room for six waves per SIMD.
+// per wave; with the second ray's sample list the pair kernel's wave holds 6 KiB), so LDS
+// leaves room for six waves per SIMD, one more than the train step's registers allow.
 template <int INTERP>
 struct ScatterItems {
   static constexpr int kPerSample = INTERP == PXL_INTERP_NEAREST ? 1 : 8;
