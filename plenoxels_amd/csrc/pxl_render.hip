@@ -692,16 +692,46 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
     const uint32_t z = z0 + (uint32_t)tid;
     const bool active = (uint32_t)tid < nz;
     const uint32_t zp = z == 0 ? (uint32_t)Z - 1u : z - 1u;
-    // occupancy bit (bx, r) at bit bx * NR + r: own for z; the lane below supplies z - 1
+    // occupancy bit (bx, r) at bit bx * NR + r: own for z; the lane below supplies z - 1 (the
+    // first lane of each wave reads it).  Every word index is formed first (rows past Y - 1 and
+    // lanes past the row's end read word 0; lanes other than 0 repeat their own word for z - 1,
+    // which coalesces), then all loads are issued before any is tested: one memory round trip.
     uint32_t own = 0u, own_p = 0u;
+    if (occ == nullptr) {
 #pragma unroll
-    for (int bx = 0; bx < 2; ++bx)
+      for (int bx = 0; bx < 2; ++bx)
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (active && rv[r] && occupied_at(occ, xrow[bx], yrow[r], z, TY, TZ)) own |= 1u << (bx * NR + r);
-        if (lane == 0 && active && rv[r] && occupied_at(occ, xrow[bx], yrow[r], zp, TY, TZ))
-          own_p |= 1u << (bx * NR + r);
-      }
+        for (int r = 0; r < NR; ++r)
+          if (active && rv[r]) own |= 1u << (bx * NR + r);
+      own_p = lane == 0 ? own : 0u;
+    } else {
+      OccBit ob[2][NR], obp[2][NR];
+      const uint32_t zc = active ? z : 0u, zpc = active && lane == 0 ? zp : zc;
+#pragma unroll
+      for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const uint32_t yc = rv[r] ? yrow[r] : 0u;
+          ob[bx][r] = occ_bit(xrow[bx], yc, zc, TY, TZ);
+          obp[bx][r] = occ_bit(xrow[bx], yc, zpc, TY, TZ);
+        }
+      uint32_t w[2][NR], wp[2][NR];
+#pragma unroll
+      for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          w[bx][r] = occ[ob[bx][r].word];
+          wp[bx][r] = occ[obp[bx][r].word];
+        }
+#pragma unroll
+      for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const bool ok = active && rv[r];
+          own |= (uint32_t)(ok && ((w[bx][r] >> ob[bx][r].shift) & 1u)) << (bx * NR + r);
+          own_p |= (uint32_t)(ok && lane == 0 && ((wp[bx][r] >> obp[bx][r].shift) & 1u)) << (bx * NR + r);
+        }
+    }
     // entry (bx, r, z) may hold sums; lanes past the row's end (z >= Z) touch nothing
     const uint32_t written = active ? own | shift_up_u32(own, own_p) : 0u;
     float4 v[NY][4];  // all loads first (independent, in flight together), then the sums
