@@ -102,8 +102,10 @@ class Setup:
         log(f"setup {time.time() - t0:.1f}s: grid {tuple(self.grid.shape)} {self.grid.dtype}, "
             f"occupancy {w.occupancy.float().mean().item():.4f}, rays {R}, S {w.num_samples}, {w.interp}")
 
-    def step(self, ev=None):
-        """One step; ev = (train start, train end, reduce start, reduce end) events or None."""
+    def step(self, ev=None, ev_reduce=None):
+        """One step; ev = (train start, train end) events around the train-step kernel, ev_reduce
+        = (start, end) around the blocks reduce, or None (each recorded event costs the
+        stream a few microseconds, so the timed loop records only the train-step pair)."""
         if not self.use_blocks:
             fill_(self.grad, 0.0)
         o, d, tgt = camera_rays(self.camera)
@@ -114,21 +116,24 @@ class Setup:
                    blocks=self.pair.blocks if self.use_blocks else None)
         if ev is not None:
             ev[1].record()
-            ev[2].record()
         if self.use_blocks:
+            if ev_reduce is not None:
+                ev_reduce[0].record()
             blocks_reduce(self.grid, self.pair.blocks, self.grad, occupancy=self.occ_bits, accumulate=False,
                           clear=self.pair.other)
+            if ev_reduce is not None:
+                ev_reduce[1].record()
             self.pair.flip()
-        if ev is not None:
-            ev[3].record()
         return self.sq.sum() / (4 * self.R)
 
 
 def timed_fused(S, steps, warmup):
+    """Wall time of `steps` steps (train-step kernel events on each), then the train-step and
+    reduce kernel durations from events."""
     for _ in range(warmup):
         S.step()
     torch.cuda.synchronize()
-    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(steps)]
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(2)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
@@ -136,7 +141,13 @@ def timed_fused(S, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms = [e[0].elapsed_time(e[1]) for e in ev]
-    red_ms = [e[2].elapsed_time(e[3]) for e in ev]
+    red_ms = [0.0]
+    if S.use_blocks:  # the reduce's duration, in steps of their own outside the timed loop
+        evr = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(2)) for _ in range(steps)]
+        for i in range(steps):
+            S.step(None, evr[i])
+        torch.cuda.synchronize()
+        red_ms = [e[0].elapsed_time(e[1]) for e in evr]
     return wall, float(np.mean(kern_ms)), float(np.min(kern_ms)), float(np.mean(red_ms))
 
 
