@@ -112,3 +112,40 @@ def test_pairs_beta_term_vs_plain(interp):
     ok, info = grad_close(gg.cpu(), go.grad)
     say("  grad", info)
     assert ok, info
+
+
+@pytest.mark.parametrize("S,storage", [(50, torch.float32), (100, torch.bfloat16), (400, torch.float32),
+                                       (400, torch.float16)])
+def test_pairs_every_chunk_count(S, storage):
+    """Train kernel instantiations for S <= 64, 128, 512 (one, two and eight 64-sample chunks
+    per ray) and half storage: per-ray outputs equal the one-ray forward kernel's bits, and the
+    gradient the forward kernel's autograd backward."""
+    say("train pairs S", S, storage)
+    R, delta = 301, 3.9 / S  # the march crosses the whole grid whatever S
+    grid, o, d, gmin, pd, cells, bits = _scene("trilinear", 0.02, R, 400 + S)
+    cfg = MarchConfig(S, delta, tuple(gmin.tolist()), pd, "trilinear", 0.0, 1e-3)
+    gd = grid.to(storage).to(DEV)
+    tgt = torch.rand(R, 4, generator=torch.Generator().manual_seed(9))
+    out = torch.empty(R, 4, device=DEV)
+    gg = torch.zeros(grid.shape, device=DEV)
+    train_step(gd, gg, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), occupancy=bits, out=out)
+    # the one-ray kernels through the C ABI, FP32 gradient whatever the storage (an autograd
+    # leaf of half dtype would round its gradient to that dtype)
+    import ctypes
+    from plenoxels_amd import _lib
+    from plenoxels_amd.render import _grid_struct, _rays_struct
+    L = _lib.lib()
+    od, dd = o.to(DEV).contiguous(), d.to(DEV).contiguous()
+    g, m, r = _grid_struct(gd, bits), cfg.to_struct(), _rays_struct(od, dd)
+    aux = torch.empty(R, int(L.pxl_aux_floats(S)), device=DEV)
+    want = torch.empty(R, 4, device=DEV)
+    st = _lib.stream_handle(DEV)
+    _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), _lib.ptr(want), _lib.ptr(aux), st))
+    assert torch.equal(out, want)
+    grad_rgba = ((2.0 / (4 * R)) * (want - tgt.to(DEV))).contiguous()
+    gb = torch.zeros(grid.shape, device=DEV)
+    _lib.check(L.pxl_render_bwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), _lib.ptr(grad_rgba),
+                                _lib.ptr(aux), _lib.ptr(gb), st))
+    ok, info = grad_close(gg.cpu(), gb.cpu())
+    say("  grad vs render_bwd", info)
+    assert ok, info
