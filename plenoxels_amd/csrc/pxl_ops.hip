@@ -461,29 +461,78 @@ __device__ __forceinline__ double sq4(const float4& a, const float4& b) {
   return (double)(dx * dx) + (double)(dy * dy) + (double)(dz * dz) + (double)(dw * dw);
 }
 
-// Cell index -> (x, y, z) in 32-bit arithmetic (grids hold at most 2^31 - 1 cells).
-__device__ __forceinline__ void cell_xyz(uint32_t L, uint32_t Y, uint32_t Z, uint32_t& x, uint32_t& y,
-                                         uint32_t& z) {
-  const uint32_t r = L / Z;
-  z = L - r * Z;
-  x = r / Y;
-  y = r - x * Y;
+// The TV kernels walk x columns: a workgroup covers 4 y rows (one wave each) x 64 z (the lanes)
+// and a slab of kTvSlab x planes, keeping each lane's x - 1, x and x + 1 cells in registers, so
+// the x neighbours (Y*Z cells apart, rarely still in cache when read in linear order) are read
+// once; the y and z neighbours are the lines the neighbouring waves and lanes load.
+constexpr int kTvSlab = 64;
+
+struct TvUnit {
+  uint32_t x0, x1, y, z;
+  int64_t col;  // cell (x, y, z) = x * Y*Z + col
+  bool active;
+};
+
+__device__ __forceinline__ TvUnit tv_unit(Dims d) {
+  const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X;
+  const uint32_t nzb = (Z + 63u) / 64u, nyb = (Y + 3u) / 4u;
+  const uint32_t u = blockIdx.x;
+  const uint32_t zb = u % nzb, yb = (u / nzb) % nyb, xs = u / (nzb * nyb);
+  TvUnit t;
+  t.y = yb * 4u + (threadIdx.x >> 6);
+  t.z = zb * 64u + (threadIdx.x & 63u);
+  t.x0 = xs * kTvSlab;
+  t.x1 = min(X, t.x0 + kTvSlab);
+  t.active = t.y < Y && t.z < Z && t.x0 < X;
+  t.col = (int64_t)t.y * Z + t.z;
+  return t;
 }
 
+__host__ __device__ inline int64_t tv_units(int64_t X, int64_t Y, int64_t Z) {
+  return ((X + kTvSlab - 1) / kTvSlab) * ((Y + 3) / 4) * ((Z + 63) / 64);
+}
+
+// sum over cells of the squared differences to the +y, +z and +x neighbours, in that order
 __global__ __launch_bounds__(256) void k_tv_sumsq(const float4* __restrict__ g, Dims d,
                                                   double* __restrict__ sum) {
   const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
+  const TvUnit t = tv_unit(d);
   double acc = 0.0;
-  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    uint32_t x, y, z;
-    cell_xyz((uint32_t)L, Y, Z, x, y, z);
-    const float4 v = g[L];
-    if (y + 1 < Y) acc += sq4(v, g[L + Z]);
-    if (z + 1 < Z) acc += sq4(v, g[L + 1]);
-    if (x + 1 < X) acc += sq4(v, g[L + YZ]);
+  if (t.active) {
+    float4 v = g[(int64_t)t.x0 * YZ + t.col];
+    for (uint32_t x = t.x0; x < t.x1; ++x) {
+      const int64_t L = (int64_t)x * YZ + t.col;
+      const float4 vx = x + 1 < X ? g[L + YZ] : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t.y + 1 < Y) acc += sq4(v, g[L + Z]);
+      if (t.z + 1 < Z) acc += sq4(v, g[L + 1]);
+      if (x + 1 < X) acc += sq4(v, vx);
+      v = vx;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(sum, acc);
+}
+
+// v - neighbour summed over +y, -y, +z, -z, +x, -x (in that order) for the cells of a column
+// walk; vm/vp are the x - 1 / x + 1 cells (read by the caller's window)
+__device__ __forceinline__ float4 tv_dir(const float4* __restrict__ g, int64_t L, const float4& v, const float4& vm,
+                                         const float4& vp, uint32_t x, uint32_t y, uint32_t z, uint32_t X,
+                                         uint32_t Y, uint32_t Z, uint32_t YZ) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto add = [&](const float4& q) {
+    acc.x += v.x - q.x;
+    acc.y += v.y - q.y;
+    acc.z += v.z - q.z;
+    acc.w += v.w - q.w;
+  };
+  if (y + 1 < Y) add(g[L + Z]);
+  if (y > 0) add(g[L - Z]);
+  if (z + 1 < Z) add(g[L + 1]);
+  if (z > 0) add(g[L - 1]);
+  if (x + 1 < X) add(vp);
+  if (x > 0) add(vm);
+  (void)YZ;
+  return acc;
 }
 
 __global__ __launch_bounds__(256) void k_tv_bwd(const float4* __restrict__ g, Dims d,
@@ -492,29 +541,22 @@ __global__ __launch_bounds__(256) void k_tv_bwd(const float4* __restrict__ g, Di
   // d/dx of weight*sqrt(S): weight / sqrt(S) * sum over neighbours of (x - neighbour)
   const float scale = weight / (float)sqrt(*sum);
   const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
-  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    uint32_t x, y, z;
-    cell_xyz((uint32_t)L, Y, Z, x, y, z);
-    const float4 v = g[L];
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t nb[6] = {y + 1 < Y ? L + Z : -1,  y > 0 ? L - Z : -1,
-                           z + 1 < Z ? L + 1 : -1,  z > 0 ? L - 1 : -1,
-                           x + 1 < X ? L + YZ : -1, x > 0 ? L - YZ : -1};
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      if (nb[q] < 0) continue;
-      const float4 u = g[nb[q]];
-      acc.x += v.x - u.x;
-      acc.y += v.y - u.y;
-      acc.z += v.z - u.z;
-      acc.w += v.w - u.w;
-    }
+  const TvUnit t = tv_unit(d);
+  if (!t.active) return;
+  float4 vm = t.x0 > 0 ? g[(int64_t)(t.x0 - 1) * YZ + t.col] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 v = g[(int64_t)t.x0 * YZ + t.col];
+  for (uint32_t x = t.x0; x < t.x1; ++x) {
+    const int64_t L = (int64_t)x * YZ + t.col;
+    const float4 vp = x + 1 < X ? g[L + YZ] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 acc = tv_dir(g, L, v, vm, vp, x, t.y, t.z, X, Y, Z, YZ);
     float4 c = grad[L];
     c.x += scale * acc.x;
     c.y += scale * acc.y;
     c.z += scale * acc.z;
     c.w += scale * acc.w;
     grad[L] = c;
+    vm = v;
+    v = vp;
   }
 }
 
@@ -599,6 +641,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // adds it (same operations, same order), then the Adam update of k_adam follows; |grad| and the
 // half copy see the total gradient and the new parameter.  The update goes to p_out, not p_in:
 // a neighbour updated in place would change the stencil of the cells read after it.
+// Traversal: the TV kernels' column walk (tv_unit), the x - 1 / x + 1 parameters in registers.
 __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in, float4* __restrict__ p_out,
                                                  const float4* __restrict__ g, float4* __restrict__ m,
                                                  float4* __restrict__ v, float4* __restrict__ gabs, Dims d,
@@ -607,34 +650,25 @@ __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in
                                                  float neg_step, void* __restrict__ half_out, int half_dtype) {
   const float scale = weight / (float)sqrt(*sum);
   const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
-  PXL_GRID_STRIDE(L, d.X * d.Y * d.Z) {
-    uint32_t x, y, z;
-    cell_xyz((uint32_t)L, Y, Z, x, y, z);
-    float4 P = p_in[L];
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t nb[6] = {y + 1 < Y ? L + Z : -1,  y > 0 ? L - Z : -1,
-                           z + 1 < Z ? L + 1 : -1,  z > 0 ? L - 1 : -1,
-                           x + 1 < X ? L + YZ : -1, x > 0 ? L - YZ : -1};
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      if (nb[q] < 0) continue;
-      const float4 u = p_in[nb[q]];
-      acc.x += P.x - u.x;
-      acc.y += P.y - u.y;
-      acc.z += P.z - u.z;
-      acc.w += P.w - u.w;
-    }
+  const TvUnit t = tv_unit(d);
+  if (!t.active) return;  // no cross-lane operations below
+  float4 Pm = t.x0 > 0 ? p_in[(int64_t)(t.x0 - 1) * YZ + t.col] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 P = p_in[(int64_t)t.x0 * YZ + t.col];
+  for (uint32_t x = t.x0; x < t.x1; ++x) {
+    const int64_t L = (int64_t)x * YZ + t.col;
+    const float4 Pp = x + 1 < X ? p_in[L + YZ] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 acc = tv_dir(p_in, L, P, Pm, Pp, x, t.y, t.z, X, Y, Z, YZ);
     float4 G = g[L];
     G.x += scale * acc.x;
     G.y += scale * acc.y;
     G.z += scale * acc.z;
     G.w += scale * acc.w;
-    float4 M = m[L], V = v[L];
-    adam_elem(P.x, G.x, M.x, V.x, w1, beta2, w2, bc2_sqrt, eps, neg_step);
-    adam_elem(P.y, G.y, M.y, V.y, w1, beta2, w2, bc2_sqrt, eps, neg_step);
-    adam_elem(P.z, G.z, M.z, V.z, w1, beta2, w2, bc2_sqrt, eps, neg_step);
-    adam_elem(P.w, G.w, M.w, V.w, w1, beta2, w2, bc2_sqrt, eps, neg_step);
-    p_out[L] = P;
+    float4 Q = P, M = m[L], V = v[L];
+    adam_elem(Q.x, G.x, M.x, V.x, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(Q.y, G.y, M.y, V.y, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(Q.z, G.z, M.z, V.z, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    adam_elem(Q.w, G.w, M.w, V.w, w1, beta2, w2, bc2_sqrt, eps, neg_step);
+    p_out[L] = Q;
     m[L] = M;
     v[L] = V;
     if (gabs) {
@@ -644,16 +678,18 @@ __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in
     }
     if (half_out) {
       if (half_dtype == PXL_DTYPE_BF16) {
-        const uint32_t lo = to_bf16_rne(P.x) | ((uint32_t)to_bf16_rne(P.y) << 16);
-        const uint32_t hi = to_bf16_rne(P.z) | ((uint32_t)to_bf16_rne(P.w) << 16);
+        const uint32_t lo = to_bf16_rne(Q.x) | ((uint32_t)to_bf16_rne(Q.y) << 16);
+        const uint32_t hi = to_bf16_rne(Q.z) | ((uint32_t)to_bf16_rne(Q.w) << 16);
         reinterpret_cast<uint2*>(half_out)[L] = make_uint2(lo, hi);
       } else {
-        store_half(half_out, half_dtype, 4 * L + 0, P.x);
-        store_half(half_out, half_dtype, 4 * L + 1, P.y);
-        store_half(half_out, half_dtype, 4 * L + 2, P.z);
-        store_half(half_out, half_dtype, 4 * L + 3, P.w);
+        store_half(half_out, half_dtype, 4 * L + 0, Q.x);
+        store_half(half_out, half_dtype, 4 * L + 1, Q.y);
+        store_half(half_out, half_dtype, 4 * L + 2, Q.z);
+        store_half(half_out, half_dtype, 4 * L + 3, Q.w);
       }
     }
+    Pm = P;
+    P = Pp;
   }
 }
 
@@ -929,7 +965,8 @@ int pxl_tv_sumsq(const float* grid, int64_t X, int64_t Y, int64_t Z, double* sum
   int rc;
   if ((rc = pxl_host::check_dims(X, Y, Z)) || (rc = need(grid && sum_sq, "grid and sum_sq required")))
     return rc;
-  hipLaunchKernelGGL(pxl::k_tv_sumsq, dim3(stride_blocks(X * Y * Z, 256, 4096)), dim3(256), 0,
+  if (pxl::tv_units(X, Y, Z) > int64_t(0x7fffffff)) return fail(PXL_E_SIZE, "grid too large for pxl_tv_sumsq");
+  hipLaunchKernelGGL(pxl::k_tv_sumsq, dim3((unsigned)pxl::tv_units(X, Y, Z)), dim3(256), 0,
                      (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq);
   return launched("pxl_tv_sumsq");
 }
@@ -942,7 +979,8 @@ int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double*
       (rc = need(((reinterpret_cast<uintptr_t>(grid) | reinterpret_cast<uintptr_t>(grad_grid)) & 15u) == 0,
                  "grid and grad_grid must be 16-byte aligned")))
     return rc;
-  hipLaunchKernelGGL(pxl::k_tv_bwd, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0,
+  if (pxl::tv_units(X, Y, Z) > int64_t(0x7fffffff)) return fail(PXL_E_SIZE, "grid too large for pxl_tv_bwd");
+  hipLaunchKernelGGL(pxl::k_tv_bwd, dim3((unsigned)pxl::tv_units(X, Y, Z)), dim3(256), 0,
                      (hipStream_t)stream, (const float4*)grid, Dims{X, Y, Z}, sum_sq, weight,
                      reinterpret_cast<float4*>(grad_grid));
   return launched("pxl_tv_bwd");
@@ -969,7 +1007,9 @@ int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad,
   const double bc2 = 1.0 - std::pow(beta2, (double)step);
   const float neg_step = (float)(-(lr / bc1));
   const float bc2_sqrt = (float)std::pow(bc2, 0.5);
-  hipLaunchKernelGGL(pxl::k_adam_tv, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0, (hipStream_t)stream,
+  const int64_t units = pxl::tv_units(X, Y, Z);
+  if (units > int64_t(0x7fffffff)) return fail(PXL_E_SIZE, "grid too large for pxl_adam_tv_step's launch");
+  hipLaunchKernelGGL(pxl::k_adam_tv, dim3((unsigned)units), dim3(256), 0, (hipStream_t)stream,
                      reinterpret_cast<const float4*>(param_in), reinterpret_cast<float4*>(param_out),
                      reinterpret_cast<const float4*>(grad), reinterpret_cast<float4*>(exp_avg),
                      reinterpret_cast<float4*>(exp_avg_sq), reinterpret_cast<float4*>(grad_abs), Dims{X, Y, Z},
