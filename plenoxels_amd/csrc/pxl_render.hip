@@ -539,12 +539,14 @@ __global__ void k_dilate(const uint32_t* __restrict__ in, uint32_t X, uint32_t Y
 // < 0 is exactly zero in the reference's chain: clamp gives alpha 0, so its colour weight and
 // colour gradients vanish, and the clamp's gradient mask is 0 for every alpha it read.  NaN
 // alpha counts as empty.
-// A workgroup covers one unit: x, a y quad (y = 4q .. 4q+3, one wave each) and 64 consecutive z
-// (the lanes, so each wave reads whole runs of cells).  Each wave's ballot holds one byte-row
-// (y & 3) of the 8 tile words along z; four lanes assemble the words from the four ballots in
-// LDS.  Units cover the padded dims (multiples of 8), so every word, padding included, is
-// written.
+// A workgroup covers a y quad (y = 4q .. 4q+3, one wave each) and 64 consecutive z (the lanes,
+// so each wave reads whole runs of cells), and walks a slab of kAlphaOccSlab x planes: with
+// DILATE, plane x's "some corner (x, y+b, z+c) has alpha >= 0" is carried from the previous
+// step, so each plane's cells are read once.  Each wave's ballot holds one byte-row (y & 3) of
+// the 8 tile words along z; eight lanes assemble the words from the four ballots in LDS.
+// Units cover the padded dims (multiples of 8), so every word, padding included, is written.
 constexpr int kAlphaOccThreads = 256;
+constexpr uint32_t kAlphaOccSlab = 32;  // a multiple of 8: slabs cover whole tiles along x
 
 template <int DT, bool DILATE>
 __global__ __launch_bounds__(kAlphaOccThreads) void k_alpha_occupancy(const void* __restrict__ grid, uint32_t X,
@@ -553,37 +555,54 @@ __global__ __launch_bounds__(kAlphaOccThreads) void k_alpha_occupancy(const void
   __shared__ uint64_t s_m[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t TX = (X + 7u) >> 3, TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
-  const uint32_t ZB = (TZ + 7u) >> 3, YQ = TY * 2u;
-  const uint64_t n_units = (uint64_t)(TX * 8u) * YQ * ZB;
+  const uint32_t ZB = (TZ + 7u) >> 3, YQ = TY * 2u, XS = (TX * 8u + kAlphaOccSlab - 1u) / kAlphaOccSlab;
+  const uint64_t n_units = (uint64_t)XS * YQ * ZB;
   for (uint64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
-    const uint32_t zb = (uint32_t)(u % ZB), yq = (uint32_t)((u / ZB) % YQ), i = (uint32_t)(u / ((uint64_t)ZB * YQ));
+    const uint32_t zb = (uint32_t)(u % ZB), yq = (uint32_t)((u / ZB) % YQ), xs = (uint32_t)(u / ((uint64_t)ZB * YQ));
     const uint32_t j = yq * 4u + (uint32_t)wave, k = zb * 64u + (uint32_t)lane;
-    bool occ = false;
-    if (i < X && j < Y && k < Z) {
+    const bool cell_ok = j < Y && k < Z;
+    const uint32_t j1 = j + 1 == Y ? 0u : j + 1, k1 = k + 1 == Z ? 0u : k + 1;
+    // any alpha >= 0 among plane i's corners (i, j+b, k+c) (DILATE) or at (i, j, k)
+    auto plane_any = [&](uint32_t i) {
       if constexpr (DILATE) {
-        const uint32_t i1 = i + 1 == X ? 0u : i + 1, j1 = j + 1 == Y ? 0u : j + 1, k1 = k + 1 == Z ? 0u : k + 1;
-        const uint32_t is[2] = {i, i1}, js[2] = {j, j1}, ks[2] = {k, k1};
+        const uint32_t js[2] = {j, j1}, ks[2] = {k, k1};
+        bool any = false;
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          occ |= load_cell<DT>(grid, (is[c >> 2] * Y + js[(c >> 1) & 1]) * Z + ks[c & 1]).w >= 0.f;
+        for (int c = 0; c < 4; ++c) any |= load_cell<DT>(grid, (i * Y + js[c >> 1]) * Z + ks[c & 1]).w >= 0.f;
+        return any;
       } else {
-        occ = load_cell<DT>(grid, (i * Y + j) * Z + k).w >= 0.f;
+        return load_cell<DT>(grid, (i * Y + j) * Z + k).w >= 0.f;
       }
-    }
-    const uint64_t m = __ballot(occ);
-    if (lane == 0) s_m[wave] = m;
-    __syncthreads();
-    if (threadIdx.x < 8u) {  // tile word of z tile 8 zb + t
-      const uint32_t t = threadIdx.x, tz = zb * 8u + t;
-      if (tz < TZ) {
-        uint32_t word = 0u;
+    };
+    const uint32_t x0 = xs * kAlphaOccSlab, x1 = min(TX * 8u, x0 + kAlphaOccSlab);
+    bool cur = DILATE && cell_ok && x0 < X ? plane_any(x0) : false;
+    for (uint32_t i = x0; i < x1; ++i) {
+      bool occ = false;
+      if (cell_ok && i < X) {
+        if constexpr (DILATE) {
+          const uint32_t i1 = i + 1 == X ? 0u : i + 1;
+          const bool nxt = plane_any(i1);
+          occ = cur | nxt;
+          cur = nxt;
+        } else {
+          occ = plane_any(i);
+        }
+      }
+      const uint64_t m = __ballot(occ);
+      if (lane == 0) s_m[wave] = m;
+      __syncthreads();
+      if (threadIdx.x < 8u) {  // tile word of z tile 8 zb + t
+        const uint32_t t = threadIdx.x, tz = zb * 8u + t;
+        if (tz < TZ) {
+          uint32_t word = 0u;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) word |= (uint32_t)((s_m[w] >> (8u * t)) & 0xffu) << (8 * w);
-        const uint32_t tile = ((i >> 3) * TY + (yq >> 1)) * TZ + tz;
-        out[(size_t)tile * 16u + (i & 7u) * 2u + (yq & 1u)] = word;
+          for (int w = 0; w < 4; ++w) word |= (uint32_t)((s_m[w] >> (8u * t)) & 0xffu) << (8 * w);
+          const uint32_t tile = ((i >> 3) * TY + (yq >> 1)) * TZ + tz;
+          out[(size_t)tile * 16u + (i & 7u) * 2u + (yq & 1u)] = word;
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -1183,8 +1202,8 @@ int pxl_alpha_occupancy(const pxl_grid* grid, int32_t dilate, uint32_t* bits, vo
   if ((rc = check_grid(grid))) return rc;
   if (!bits) return fail(PXL_E_INVALID, "bits is NULL");
   if ((rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
-  const int64_t units = ((grid->dims[0] + 7) / 8 * 8) * ((grid->dims[1] + 7) / 8 * 2) *
-                        (((grid->dims[2] + 7) / 8 + 7) / 8);
+  const int64_t units = (((grid->dims[0] + 7) / 8 * 8 + pxl::kAlphaOccSlab - 1) / pxl::kAlphaOccSlab) *
+                        ((grid->dims[1] + 7) / 8 * 2) * (((grid->dims[2] + 7) / 8 + 7) / 8);
   const unsigned nb = (unsigned)std::min<int64_t>(units, int64_t(1) << 20);  // workgroup per unit
   hipStream_t s = (hipStream_t)stream;
   const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
