@@ -22,7 +22,8 @@ from . import _lib
 from .checkpoint import save_grid
 from .grid_functions import average_pool3d_grid, pool_backward_
 from .optim import adam_step_, adam_tv_step_, tv_backward_, tv_sumsq
-from .render import CameraRays, CornerBlockPair, MarchConfig, alpha_occupancy, beta_term_zero, fill_, train_step
+from .render import (CameraRays, CornerBlockPair, MarchConfig, alpha_occupancy, beta_term_zero, blocks_reduce, fill_,
+                     train_step)
 
 RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
 STEPS_PER_FIELD = 5                                              # scripts/train.py:92
@@ -34,6 +35,11 @@ def grid_coordinate(i, size, points_distance):
     c = np.float32(i) - np.float32(math.ceil(size / 2))
     c = c + np.float32(1)
     return float(c * np.float32(points_distance))
+
+
+# Pooled trilinear grids up to this many cells scatter through corner blocks (two 64-byte
+# entries per cell and buffer: 128 MiB each at the limit).
+POOLED_BLOCKS_MAX_CELLS = 128 ** 3
 
 
 def receptive_field(i):
@@ -54,6 +60,8 @@ class Trainer:
     corner_blocks (trilinear only) routes the gradient through corner blocks (a 64-byte entry
     per cell) and their reduce (pxl_train_step_blocks), which replaces the zero-fill; it costs
     two such buffers, 128 B per cell of device memory, and a dense reduce pass per step.
+    Pooled trilinear grids of up to POOLED_BLOCKS_MAX_CELLS cells use corner blocks in any case
+    (reduced and cleared densely, so the occupancy may change from step to step).
     skip_empty (default on) renders each step with the occupancy of the cells whose raw alpha
     is >= 0 (pxl_alpha_occupancy): the samples it skips are exactly zero in the reference chain
     (clamped alpha 0, zero clamp gradient), so finite results are unchanged while the empty
@@ -103,6 +111,7 @@ class Trainer:
         self.bsum = torch.empty(n, device=self.dev)
         self.tvs = torch.zeros(1, dtype=torch.float64, device=self.dev)
         self._grid_next = None  # second FP32 grid of the fused TV + Adam update (allocated on first use)
+        self._pooled_pair = None  # (pooled dims, CornerBlockPair) of the current receptive field
         if self.even_spread:
             m = int(round(math.sqrt(self.R)))
             ls = torch.linspace(0, end=1, steps=m, device=self.dev)
@@ -126,8 +135,14 @@ class Trainer:
             gmin = tuple(grid_coordinate(start, s, self.pd) for s in self.dims)
             pd = self.pd * stride
             grid = average_pool3d_grid(self.grid, receptive_field_size=k, stride=stride).detach()
-            grad = torch.empty_like(grid) if self.use_blocks else torch.zeros_like(grid)
+            # small pooled grids go through corner blocks (half the atomic requests of the direct
+            # scatter, which bounds these steps while every sample is still kept), reduced and
+            # cleared densely so the per-step occupancy may change
+            pooled_blocks = (not self.use_blocks and self.interp == "trilinear"
+                             and grid.shape[0] * grid.shape[1] * grid.shape[2] <= POOLED_BLOCKS_MAX_CELLS)
+            grad = torch.empty_like(grid) if self.use_blocks or pooled_blocks else torch.zeros_like(grid)
         else:
+            pooled_blocks = False
             gmin = tuple(grid_coordinate(0, s, self.pd) for s in self.dims)
             pd = self.pd
             grid = self.grid if self.render_copy is None else self.render_copy
@@ -145,10 +160,20 @@ class Trainer:
                 self._blocks[key] = CornerBlockPair(grid)
             pair = self._blocks[key]
         occ = alpha_occupancy(grid, self.interp) if self.skip_empty else None
-        train_step(grid, grad, cfg=cfg, camera=cam, occupancy=occ, ray_sq_err=self.sq,
-                   beta=self.beta if self.beta > 0 else 0.0,
-                   ray_beta=self.bsum if self.beta > 0 else None, blocks=pair.blocks if pair else None,
-                   overwrite=True, clear=pair.other if pair else None)
+        if pooled_blocks:
+            key = tuple(grid.shape[:3])
+            if self._pooled_pair is None or self._pooled_pair[0] != key:
+                self._pooled_pair = (key, CornerBlockPair(grid))
+            pair = self._pooled_pair[1]
+            train_step(grid, None, cfg=cfg, camera=cam, occupancy=occ, ray_sq_err=self.sq,
+                       beta=self.beta if self.beta > 0 else 0.0, ray_beta=self.bsum if self.beta > 0 else None,
+                       blocks=pair.blocks)
+            blocks_reduce(grid, pair.blocks, grad, occupancy=None, accumulate=False, clear=pair.other)
+        else:
+            train_step(grid, grad, cfg=cfg, camera=cam, occupancy=occ, ray_sq_err=self.sq,
+                       beta=self.beta if self.beta > 0 else 0.0,
+                       ray_beta=self.bsum if self.beta > 0 else None, blocks=pair.blocks if pair else None,
+                       overwrite=True, clear=pair.other if pair else None)
         if pair:
             pair.flip()
         closs = self.sq.sum() / (4 * n_rays)
