@@ -134,6 +134,20 @@ __device__ __forceinline__ int lane_rank(uint64_t m) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Streaming 16-byte accesses (the `nt` cache policy): for data touched once per pass -- dense
+// gradient rows, cleared block entries, image pixels -- so that it does not evict the lines
+// other waves are about to reuse from L2 and the memory-side cache.
+typedef float float4_nt __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 load_stream(const float4* p) {
+  const float4_nt t = __builtin_nontemporal_load(reinterpret_cast<const float4_nt*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+  __builtin_nontemporal_store((float4_nt){v.x, v.y, v.z, v.w}, reinterpret_cast<float4_nt*>(p));
+}
+
 // ----------------------------------------------------------------------------- grid cells
 
 template <int DT>
@@ -279,7 +293,7 @@ __device__ __forceinline__ float4 camera_target(const KParams& P, const Ray& R) 
   col = fminf(fmaxf(col, 0.f), W - 1.f);
   row = fminf(fmaxf(row, 0.f), H - 1.f);
   const int64_t pix = ((int64_t)R.cam * P.img_h + (int64_t)row) * P.img_w + (int64_t)col;
-  return reinterpret_cast<const float4*>(P.imgs)[pix];
+  return load_stream(reinterpret_cast<const float4*>(P.imgs) + pix);
 }
 
 // Both forms produce a whole Ray value that is assigned once: branches that stored to
