@@ -13,7 +13,7 @@ lookup, occupancy skipping, early termination at transmittance 1e-4.  Synthetic,
 One step = pxl_camera_rays (ray generation from (camera, u, v) and target gather) +
 pxl_train_step_blocks (sample placement, gather, compositing, MSE gradient, backward scatter
 into per-cell corner-block entries) + pxl_blocks_reduce (writes the FP32 (X,Y,Z,4) grid gradient -- no separate
-zero-fill -- and zeroes the other of two alternating block buffers) + the loss reduction.  --scatter direct times the
+zero-fill -- and zeroes the other of two alternating block buffers) + the loss value (pxl_scaled_sum).  --scatter direct times the
 zero-fill + pxl_train_step with atomics straight into the gradient instead (nearest always
 scatters directly: one cell per sample).  Adam is not part of the step (reference
 scripts/train.py:182 runs it after the timed chain).
@@ -37,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from plenoxels_amd import (CameraRays, CornerBlockPair, MarchConfig, _lib, blocks_reduce,  # noqa: E402
+from plenoxels_amd import (CameraRays, CornerBlockPair, LossMean, MarchConfig, _lib, blocks_reduce,  # noqa: E402
                            camera_rays, fill_, occupancy_dilate, pack_occupancy, train_step)
 from plenoxels_amd import scenes  # noqa: E402
 
@@ -98,6 +98,7 @@ class Setup:
         self.pair = CornerBlockPair(self.grid) if self.use_blocks else None
         self.out = torch.empty(R, 4, device=dev)
         self.sq = torch.empty(R, device=dev)
+        self.loss_mean = LossMean(dev)
         torch.cuda.synchronize()
         log(f"setup {time.time() - t0:.1f}s: grid {tuple(self.grid.shape)} {self.grid.dtype}, "
             f"occupancy {w.occupancy.float().mean().item():.4f}, rays {R}, S {w.num_samples}, {w.interp}")
@@ -124,7 +125,7 @@ class Setup:
             if ev_reduce is not None:
                 ev_reduce[1].record()
             self.pair.flip()
-        return self.sq.sum() / (4 * self.R)
+        return self.loss_mean(self.sq)
 
 
 def timed_fused(S, steps, warmup):

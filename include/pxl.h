@@ -144,6 +144,16 @@ typedef struct pxl_loss {
 int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
                    const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream);
 
+/* A step's loss value from its per-ray squared errors: out[0] = (float)(scale * sum of
+ * v[0, n)), summed in FP64 in a fixed order (the same bits on every call), in one launch.
+ * With v = pxl_loss.ray_sq_err and scale = 1 / (4 * N_rays) this is mse_loss's mean
+ * (scripts/train.py:156), the value the reference reads every step (:159).  v and work
+ * 16-byte aligned; work: pxl_scaled_sum_work_bytes() bytes, zeroed before its first use and
+ * left ready for the next call by every call (calls sharing one work buffer must not overlap:
+ * keep them on one stream). */
+int pxl_scaled_sum(const float* v, int64_t n, double scale, float* out, void* work, void* stream);
+int64_t pxl_scaled_sum_work_bytes(void);
+
 /* pxl_train_step with the trilinear gradient accumulated into corner blocks instead of
  * grad_grid: blocks is (X*Y*Z, 16) f32 (pxl_blocks_floats), zero on entry: one 64-byte entry
  * per cell E = (x, y, z), whose slot s = 2bx + by (floats 4s + c) receives channel c of corner

@@ -693,6 +693,57 @@ __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in
   }
 }
 
+// out[0] = (float)(scale * sum of v[0, n)) in one launch and the same bits on every call.
+// Workgroup b sums, in FP64 and a fixed order, the float4s b*256 + t + k*(gridDim*256) and
+// writes partial[b]; the last workgroup to take a ticket adds the partials in index order
+// with a fixed tree, writes out and puts the ticket back to zero for the next call.
+constexpr int kSumBlocks = 64;
+
+__device__ __forceinline__ double wave_sum_f64(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_scaled_sum(const float4* __restrict__ v4, const float* __restrict__ v,
+                                                    int64_t n, double scale, float* __restrict__ out,
+                                                    double* __restrict__ partial, uint32_t* __restrict__ ticket) {
+  __shared__ double wsum[4];
+  __shared__ uint32_t last;
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * 256;
+  int64_t j = (int64_t)blockIdx.x * 256 + tid;
+  double s = 0.0;
+  for (; j + 3 * stride < n4; j += 4 * stride) {  // four 16-byte loads in flight
+    const float4 a = v4[j], b = v4[j + stride], c = v4[j + 2 * stride], d = v4[j + 3 * stride];
+    s += (((double)a.x + a.y) + ((double)a.z + a.w)) + (((double)b.x + b.y) + ((double)b.z + b.w));
+    s += (((double)c.x + c.y) + ((double)c.z + c.w)) + (((double)d.x + d.y) + ((double)d.z + d.w));
+  }
+  for (; j < n4; j += stride) {
+    const float4 a = v4[j];
+    s += ((double)a.x + a.y) + ((double)a.z + a.w);
+  }
+  if (blockIdx.x == 0 && tid < (int)(n & 3)) s += (double)v[4 * n4 + tid];
+  s = wave_sum_f64(s);
+  if (lane == 0) wsum[wave] = s;
+  __syncthreads();
+  if (tid == 0) {
+    partial[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    __threadfence();  // the partial is visible device-wide before the ticket is taken
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+  }
+  __syncthreads();
+  if (last == 0u || wave != 0) return;
+  __threadfence();
+  double p = lane < (int)gridDim.x ? __hip_atomic_load(partial + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : 0.0;
+  p = wave_sum_f64(p);
+  if (lane == 0) {
+    out[0] = (float)(p * scale);
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace
 }  // namespace pxl
 
@@ -959,6 +1010,21 @@ int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, i
   hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0, st,
                      (const float4*)d, X * Y, Z, (int64_t)1, k, s, oz, 1, (float4*)grad_grid);
   return launched("pxl_avg_pool3d_bwd");
+}
+
+int64_t pxl_scaled_sum_work_bytes(void) { return 1024; }
+
+int pxl_scaled_sum(const float* v, int64_t n, double scale, float* out, void* work, void* stream) {
+  if (n < 0) return fail(PXL_E_INVALID, "n must be >= 0");
+  if (!out || !work || (n > 0 && !v)) return fail(PXL_E_INVALID, "v, out and work are required");
+  if (reinterpret_cast<uintptr_t>(v) % 16 != 0 || reinterpret_cast<uintptr_t>(work) % 16 != 0)
+    return fail(PXL_E_INVALID, "v and work must be 16-byte aligned");
+  static_assert(pxl::kSumBlocks * sizeof(double) + sizeof(uint32_t) <= 1024, "work layout");
+  double* partial = reinterpret_cast<double*>(work);
+  uint32_t* ticket = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(work) + pxl::kSumBlocks * sizeof(double));
+  hipLaunchKernelGGL(pxl::k_scaled_sum, dim3(pxl::kSumBlocks), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(v), v, n, scale, out, partial, ticket);
+  return launched("pxl_scaled_sum");
 }
 
 int pxl_tv_sumsq(const float* grid, int64_t X, int64_t Y, int64_t Z, double* sum_sq, void* stream) {

@@ -357,6 +357,29 @@ def beta_term_zero():
     return float(np.log(np.float32(BETA_EPS)) - np.log(np.float32(1.0 + BETA_EPS)))
 
 
+class LossMean:
+    """mse_loss's mean (scripts/train.py:156) from per-ray squared errors (train_step's
+    ray_sq_err) on the device: scale * sum in FP64, one launch, the same bits on every call
+    (pxl_scaled_sum).  Holds the call's small work buffer: use one instance per stream."""
+
+    def __init__(self, device):
+        n = int(_lib.lib().pxl_scaled_sum_work_bytes())
+        self.work = torch.zeros(n, dtype=torch.uint8, device=device)
+
+    def __call__(self, ray_sq_err, n_rays=None):
+        """0-dim float32 tensor: sum(ray_sq_err) / (4 * n_rays) (n_rays defaults to its length)."""
+        v = ray_sq_err
+        if v.dtype != torch.float32 or v.dim() != 1 or not v.is_contiguous() or v.device != self.work.device:
+            raise ValueError("ray_sq_err must be a contiguous 1-D float32 tensor on the work buffer's device")
+        n_rays = v.shape[0] if n_rays is None else int(n_rays)
+        if n_rays <= 0:
+            raise ValueError("n_rays must be positive")
+        out = torch.empty((), dtype=torch.float32, device=v.device)
+        _lib.check(_lib.lib().pxl_scaled_sum(_lib.ptr(v), v.shape[0], 1.0 / (4.0 * n_rays), _lib.ptr(out),
+                                             _lib.ptr(self.work), _lib.stream_handle(v.device)))
+        return out
+
+
 def fill_(t, value=0.0):
     """In-place fill of a contiguous float32 device tensor with the library's fill kernel."""
     if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:

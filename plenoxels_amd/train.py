@@ -22,8 +22,8 @@ from . import _lib
 from .checkpoint import save_grid
 from .grid_functions import average_pool3d_grid, pool_backward_
 from .optim import adam_step_, adam_tv_step_, tv_backward_, tv_sumsq
-from .render import (CameraRays, CornerBlockPair, MarchConfig, alpha_occupancy, beta_term_zero, blocks_reduce, fill_,
-                     train_step)
+from .render import (CameraRays, CornerBlockPair, LossMean, MarchConfig, alpha_occupancy, beta_term_zero, blocks_reduce,
+                     fill_, train_step)
 
 RECEPTIVE_FIELDS = [i for i in range(93, 2, -2) if i % 2 != 0]  # scripts/train.py:91
 STEPS_PER_FIELD = 5                                              # scripts/train.py:92
@@ -108,6 +108,7 @@ class Trainer:
         n = self.B * self.R
         self.cam_ids = torch.repeat_interleave(torch.arange(self.B, dtype=torch.int32, device=self.dev), self.R)
         self.sq = torch.empty(n, device=self.dev)
+        self.loss_mean = LossMean(self.dev)
         self.bsum = torch.empty(n, device=self.dev)
         self.tvs = torch.zeros(1, dtype=torch.float64, device=self.dev)
         self._grid_next = None  # second FP32 grid of the fused TV + Adam update (allocated on first use)
@@ -176,7 +177,7 @@ class Trainer:
                        overwrite=True, clear=pair.other if pair else None)
         if pair:
             pair.flip()
-        closs = self.sq.sum() / (4 * n_rays)
+        closs = self.loss_mean(self.sq, n_rays)
         out = {"closs": closs}
         loss = closs
         # full-grid steps with the TV term fold its backward into the Adam pass (pxl_adam_tv_step,

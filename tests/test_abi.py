@@ -209,6 +209,19 @@ def test_corner_block_validation_before_any_hip_call():
     assert L.pxl_alpha_occupancy(None, 1, ctypes.c_void_p(0x10), None) == _lib.PXL_E_INVALID
 
 
+def test_scaled_sum_validation():
+    L = _lib.lib()
+    assert L.pxl_scaled_sum_work_bytes() == 1024
+    work, out = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), -1, 1.0, out, work, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum(None, 5, 1.0, out, work, None) == _lib.PXL_E_INVALID  # n > 0 needs v
+    assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), 5, 1.0, None, work, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), 5, 1.0, out, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum(ctypes.c_void_p(0x3004), 5, 1.0, out, work, None) == _lib.PXL_E_INVALID
+    assert "aligned" in _lib.last_error()
+    assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), 5, 1.0, out, ctypes.c_void_p(0x1008), None) == _lib.PXL_E_INVALID
+
+
 def test_no_device_is_reported_not_aborted():
     import torch
     if torch.cuda.is_available():

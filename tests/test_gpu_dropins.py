@@ -326,3 +326,35 @@ def test_trainer_matches_reference_loop(start, dims, progressive):
     say("  grid: max diff", diff.max().item(), "frac > 1e-5:", (diff > 1e-5).float().mean().item())
     assert (diff > 1e-5).float().mean().item() <= 1e-3
     assert diff.max().item() <= 2 * 0.01 * len(uniforms)
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 5, 1000, 4099, 262144, (1 << 20) + 3])
+def test_scaled_sum_matches_exact_sum(n):
+    """pxl_scaled_sum (the loss value of a step, mse_loss's mean scripts/train.py:156, from the
+    per-ray squared errors): float32(scale * sum) within one float32 ulp of the exactly rounded
+    value (math.fsum), ragged lengths (tails of 1-3 floats, fewer float4s than threads), the
+    same bits on every call, and the work buffer's ticket left ready for calls of other lengths."""
+    import ctypes
+    from plenoxels_amd import LossMean, _lib
+    say("scaled sum n", n)
+    L = _lib.lib()
+    v = (torch.rand(n, generator=torch.Generator().manual_seed(n + 7), dtype=torch.float64) * 3).float()
+    vd = v.to(DEV)
+    work = torch.zeros(int(L.pxl_scaled_sum_work_bytes()), dtype=torch.uint8, device=DEV)
+    small = torch.rand(7, generator=torch.Generator().manual_seed(1)).to(DEV)
+    scale = 0.37
+    outs = []
+    for k in range(3):
+        out = torch.full((), -1.0, device=DEV)
+        _lib.check(L.pxl_scaled_sum(_lib.ptr(vd), n, scale, _lib.ptr(out), _lib.ptr(work), _lib.stream_handle(DEV)))
+        outs.append(out.item())
+        o2 = torch.empty((), device=DEV)  # a call of another length in between
+        _lib.check(L.pxl_scaled_sum(_lib.ptr(small), 7, 1.0, _lib.ptr(o2), _lib.ptr(work), _lib.stream_handle(DEV)))
+        assert o2.item() == float(np.float32(math.fsum(small.cpu().double().tolist())))
+    exact = scale * math.fsum(v.double().tolist())
+    want = float(np.float32(exact))
+    assert outs[0] == outs[1] == outs[2]
+    assert abs(outs[0] - want) <= 2.0 ** -23 * abs(want), (outs[0], want)
+    if n > 0:
+        lm = LossMean(DEV)(vd)
+        assert abs(lm.item() - math.fsum(v.double().tolist()) / (4 * n)) <= 2.0 ** -23 * lm.item()
