@@ -667,10 +667,11 @@ __global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ o
 // that the workgroups in flight on one XCD (workgroup id mod 8) hold neighbouring y rows,
 // which read the same entries.  `clear` (a second block buffer, not read in this pass) has the
 // written entries of each output row zeroed in the same pass by coalesced stores, ready for
-// the next step.  Gradient rows, cleared entries and entry loads use the streaming cache
-// policy: written (or read) once here, they would otherwise evict the entry lines that the
-// workgroup of the next x row reads (measured: 167 -> 88 us for this pass alone at 256^3 with
-// ~5% occupancy, and the following train step's gathers run faster too).
+// the next step.  Gradient rows and cleared entries are stored with the streaming cache
+// policy: written once here, they would otherwise evict the entry lines that the workgroup of
+// the next x row reads (measured: 167 -> 88 us for this pass alone at 256^3 with ~5%
+// occupancy, and the following train step's gathers run faster too).  Entry loads stay
+// ordinary for that reuse.
 constexpr int kReduceCells = 256;
 constexpr int kReduceRows = 4;  // NY
 
@@ -764,7 +765,7 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
         const int bx = m >> 1, r = i + (m & 1);
         v[i][m] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (y0 + (uint32_t)i < (uint32_t)Y && ((written >> (bx * NR + r)) & 1u))
-          v[i][m] = load_stream(blocks + (size_t)(base[bx][r] + z) * 4u + m);
+          v[i][m] = blocks[(size_t)(base[bx][r] + z) * 4u + m];
       }
 #pragma unroll
     for (int i = 0; i < NY; ++i) {
