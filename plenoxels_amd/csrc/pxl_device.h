@@ -587,7 +587,13 @@ __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int 
 // Scatter one chunk's gradients: kept lanes stage their items in this wave's LDS slice
 // (compacted by rank), then the whole wave flushes them four lanes per cell, so one atomic
 // instruction covers 16 cells and each cell's four channels go out as one 16-byte group.
-template <int INTERP>
+// With RUNS, trilinear stages where at least a quarter of the samples share the previous
+// sample's floor cell (coarse grids: the pooled grids of progressive growing) are flushed by
+// runs instead: a sample's eight item indices depend only on its floor cell (item 0
+// identifies it), so each half-wave walks its 16 staged samples with one lane per (corner,
+// channel), sums a run of same-cell samples in sample order and adds the sum once -- two
+// memory requests per run instead of per sample.
+template <int INTERP, bool RUNS = false>
 __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, uint32_t clip, const float4& gv,
                                               float* grad, uint32_t* s_cell, float4* s_val) {
@@ -602,12 +608,30 @@ __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, in
   for (int h = 0; h < nk; h += NS) {  // kept samples of rank h .. h+NS-1 per stage
     if (keep && rank >= h && rank < h + NS) write_items<INTERP>(P, R, k, clip, gv, s_cell, s_val, (rank - h) * NI);
     wave_lds_sync();
-    const int n = min(nk - h, NS) * NI;
-    for (int base = 0; base < n; base += kWave / 4) {
-      const int it = base + (lane >> 2);
-      if (it < n) {
-        const float v = sv[4 * it + ch];
-        if (v != 0.f) atomic_add_f32(grad + (size_t)s_cell[it] * 4u + ch, v);
+    const int ns = min(nk - h, NS), n = ns * NI;
+    bool by_runs = false;
+    if constexpr (RUNS && INTERP == PXL_INTERP_TRILINEAR) {
+      const bool dup = lane >= 1 && lane < ns && s_cell[lane * NI] == s_cell[(lane - 1) * NI];
+      by_runs = 4 * __popcll(__ballot(dup)) >= ns;
+      if (by_runs) {
+        const int j = (lane >> 2) & 7, r0 = (lane >> 5) * (NS / 2), r1 = min(ns, r0 + NS / 2);
+        float acc = 0.f;
+        for (int r = r0; r < r1; ++r) {
+          acc += sv[4 * (r * NI + j) + ch];
+          if (r + 1 == r1 || s_cell[(r + 1) * NI] != s_cell[r * NI]) {  // end of a run
+            if (acc != 0.f) atomic_add_f32(grad + (size_t)s_cell[r * NI + j] * 4u + ch, acc);
+            acc = 0.f;
+          }
+        }
+      }
+    }
+    if (!by_runs) {
+      for (int base = 0; base < n; base += kWave / 4) {
+        const int it = base + (lane >> 2);
+        if (it < n) {
+          const float v = sv[4 * it + ch];
+          if (v != 0.f) atomic_add_f32(grad + (size_t)s_cell[it] * 4u + ch, v);
+        }
       }
     }
     wave_lds_sync();

@@ -135,7 +135,7 @@ __device__ __forceinline__ float beta_dloss(float a) {
 // One group of the reverse sweep: given the group's clipped values, transmittance and Qn
 // (the suffix value just behind the group), emit dL/dvalue per sample, scatter it, and
 // return in Qn the suffix value in front of the group.
-template <int INTERP>
+template <int INTERP, bool RUNS = false>
 __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, const float4& v, uint32_t clip,
                                               float Ti, const float4& g, float beta_scale,
@@ -167,7 +167,7 @@ __device__ __forceinline__ void reverse_chunk(const KParams& P, const Ray& R, in
     gv.w = Ti * (g.x * (v.x - s0) + g.y * (v.y - s1) + g.z * (v.z - s2) + g.w * (1.f - s3));
     if (beta_scale != 0.f) gv.w += beta_scale * beta_dloss(v.w);
   }
-  scatter_chunk<INTERP>(P, R, lane, k, keep, clip, gv, grad, s_cell, s_val);
+  scatter_chunk<INTERP, RUNS>(P, R, lane, k, keep, clip, gv, grad, s_cell, s_val);
   Qn = make_float4(readlane(f.a0, 0), readlane(f.a1, 0), readlane(f.a2, 0), readlane(f.a3, 0));
 }
 
@@ -245,6 +245,12 @@ struct LossParams {
 // The train step of one ray after classification: its kept samples (list[0..nk)) in groups of
 // 64, loss and its gradient, the reverse sweep and the scatter.  `list` may alias the staging
 // area (s_cell, s_val): it is read completely before the first scatter.
+// The run flush of scatter_chunk goes into the train kernels for more than 256 samples per ray
+// (three waves per SIMD, registers to spare): the kernels for up to 256 samples keep five
+// waves per SIMD only without it (with it they spill or lose a wave).
+template <int NG>
+constexpr bool kRunFlush = NG > 4;
+
 template <int DT, int INTERP, int NG>
 __device__ __forceinline__ void train_ray(const KParams& P, const LossParams& LP, const Ray& R,
                                           int64_t ray, const float4& tgt, const int* list, int nk,
@@ -305,7 +311,7 @@ __device__ __forceinline__ void train_ray(const KParams& P, const LossParams& LP
     // a group with no processed sample is the identity map: Qn passes through unchanged
     if (g < ng && KEEP[g] != 0ull) {
       const bool keep = (KEEP[g] >> lane) & 1ull;
-      reverse_chunk<INTERP>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, LP.beta_scale, Qn,
+      reverse_chunk<INTERP, kRunFlush<NG>>(P, R, lane, KI[g], keep, V[g], CL[g], TI[g], gl, LP.beta_scale, Qn,
                             grad, s_cell, s_val);
     }
   }
@@ -368,7 +374,7 @@ __device__ __forceinline__ Ray select_ray(bool second, const Ray& a, const Ray& 
 
 // Rays ray0 (lanes 0-31, list0[0..n0)) and ray0 + 1 (lanes 32-63, list1[0..n1)), n0, n1 <= 32,
 // one group of each; the steps of train_ray with T = 1 and Qn = 0.
-template <int DT, int INTERP>
+template <int DT, int INTERP, int NG>
 __device__ __forceinline__ void train_half_pair(const KParams& P, const LossParams& LP,
                                                 const Ray& R0, const Ray& R1, int64_t ray0,
                                                 bool has1, const float4& tgt0, const float4& tgt1,
@@ -444,7 +450,7 @@ __device__ __forceinline__ void train_half_pair(const KParams& P, const LossPara
     gv.w = Ti * (gl.x * (V.x - s0) + gl.y * (V.y - s1) + gl.z * (V.z - s2) + gl.w * (1.f - s3));
     if (LP.beta_scale != 0.f) gv.w += LP.beta_scale * beta_dloss(V.w);
   }
-  scatter_chunk<INTERP>(P, R, lane, k, proc, CL, gv, grad, s_cell, s_val);
+  scatter_chunk<INTERP, kRunFlush<NG>>(P, R, lane, k, proc, CL, gv, grad, s_cell, s_val);
 }
 
 template <int DT, int INTERP, int NG>
@@ -486,7 +492,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG <= 4 
   }
   wave_lds_sync();
   if (n0 <= 32 && n1 <= 32) {
-    train_half_pair<DT, INTERP>(P, LP, R0, R1, ray0, has1, tgt0, tgt1, list0, n0, list1, n1, lane,
+    train_half_pair<DT, INTERP, NG>(P, LP, R0, R1, ray0, has1, tgt0, tgt1, list0, n0, list1, n1, lane,
                                 L::cells(mem), L::vals(mem), out, grad);
   } else {
     train_ray<DT, INTERP, NG>(P, LP, R0, ray0, tgt0, list0, n0, lane, L::cells(mem), L::vals(mem), out, grad);
