@@ -324,7 +324,7 @@ __device__ __forceinline__ void train_ray(const KParams& P, const LossParams& LP
 // half_sums), which evaluates, for each half, the same operation tree the one-ray kernel uses
 // on lanes 0-31 -- so every value is bit-identical to the one-ray path's (and to
 // k_render_fwd's).  Otherwise the two rays run one after the other through the one-ray path
-// (train_ray).  The register budget is capped at five waves per SIMD (95 VGPRs for trilinear,
+// (train_ray).  The register budget is capped at five waves per SIMD (96 VGPRs for trilinear,
 // S <= 256), which the one-ray path alone needed; S > 256 and FP16 trilinear would spill.
 __device__ __forceinline__ float half_prefix_mul(float x) {
   x *= dpp<DPP_ROW_SHR1>(1.f, x);
