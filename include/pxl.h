@@ -328,11 +328,15 @@ int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_av
  * (grad itself is not written), and drives the Adam update, the |grad| accumulation and the
  * half copy; the result equals the two separate calls bit for bit.  The updated parameter is
  * written to param_out, which must be a different buffer from param_in: the TV stencil reads
- * each cell's six neighbours from param_in.  Arrays 16-byte aligned (half_out 8-byte). */
-int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad, float* exp_avg,
+ * each cell's six neighbours from param_in.  grad is not written unless zero_grad is nonzero:
+ * then every grad element read as nonzero is set to 0 after use, so grad is all zero on
+ * return and the next step's scatter needs no zero-fill (a step's gradient covers few cells).
+ * Arrays 16-byte aligned (half_out 8-byte). */
+int pxl_adam_tv_step(const float* param_in, float* param_out, float* grad, float* exp_avg,
                      float* exp_avg_sq, float* grad_abs, int64_t X, int64_t Y, int64_t Z,
                      const double* tv_sum_sq, float tv_weight, double lr, double beta1, double beta2,
-                     double eps, int64_t step, void* half_out, int32_t half_dtype, void* stream);
+                     double eps, int64_t step, void* half_out, int32_t half_dtype, int32_t zero_grad,
+                     void* stream);
 
 /* Host-side traffic accounting (no counterpart in the reference; ALL POINTERS ARE HOST
  * POINTERS, including grid->occupancy, and grid->data is not read): counts[r] = number of

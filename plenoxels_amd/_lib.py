@@ -121,7 +121,7 @@ SIGNATURES = {
     "pxl_adam_tv_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p, ctypes.c_float,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                        ctypes.c_int64, c_void_p, ctypes.c_int32, c_void_p]),
+                                        ctypes.c_int64, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p]),
     "pxl_count_cells_host": (ctypes.c_int, [_GP, _MP, c_void_p, c_void_p, ctypes.c_int64,
                                             c_void_p, c_void_p, ctypes.c_int32]),
     "pxl_debug_wave_ops": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),

@@ -643,11 +643,12 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // a neighbour updated in place would change the stencil of the cells read after it.
 // Traversal: the TV kernels' column walk (tv_unit), the x - 1 / x + 1 parameters in registers.
 __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in, float4* __restrict__ p_out,
-                                                 const float4* __restrict__ g, float4* __restrict__ m,
+                                                 float4* __restrict__ g, float4* __restrict__ m,
                                                  float4* __restrict__ v, float4* __restrict__ gabs, Dims d,
                                                  const double* __restrict__ sum, float weight, float w1,
                                                  float beta2, float w2, float bc2_sqrt, float eps,
-                                                 float neg_step, void* __restrict__ half_out, int half_dtype) {
+                                                 float neg_step, void* __restrict__ half_out, int half_dtype,
+                                                 int zero_grad) {
   const float scale = weight / (float)sqrt(*sum);
   const uint32_t Y = (uint32_t)d.Y, Z = (uint32_t)d.Z, X = (uint32_t)d.X, YZ = Y * Z;
   const TvUnit t = tv_unit(d);
@@ -658,7 +659,11 @@ __global__ __launch_bounds__(256) void k_adam_tv(const float4* __restrict__ p_in
     const int64_t L = (int64_t)x * YZ + t.col;
     const float4 Pp = x + 1 < X ? p_in[L + YZ] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 acc = tv_dir(p_in, L, P, Pm, Pp, x, t.y, t.z, X, Y, Z, YZ);
-    float4 G = g[L];
+    const float4 G0 = g[L];
+    // zero_grad: the scatter's cells go back to zero once read (most cells already are)
+    if (zero_grad && (G0.x != 0.f || G0.y != 0.f || G0.z != 0.f || G0.w != 0.f))
+      store_stream(g + L, make_float4(0.f, 0.f, 0.f, 0.f));
+    float4 G = G0;
     G.x += scale * acc.x;
     G.y += scale * acc.y;
     G.z += scale * acc.z;
@@ -1052,10 +1057,11 @@ int pxl_tv_bwd(const float* grid, int64_t X, int64_t Y, int64_t Z, const double*
   return launched("pxl_tv_bwd");
 }
 
-int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad, float* exp_avg,
+int pxl_adam_tv_step(const float* param_in, float* param_out, float* grad, float* exp_avg,
                      float* exp_avg_sq, float* grad_abs, int64_t X, int64_t Y, int64_t Z,
                      const double* tv_sum_sq, float tv_weight, double lr, double beta1, double beta2,
-                     double eps, int64_t step, void* half_out, int32_t half_dtype, void* stream) {
+                     double eps, int64_t step, void* half_out, int32_t half_dtype, int32_t zero_grad,
+                     void* stream) {
   int rc;
   auto al16 = [](const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
   if ((rc = pxl_host::check_dims(X, Y, Z)) ||
@@ -1077,10 +1083,10 @@ int pxl_adam_tv_step(const float* param_in, float* param_out, const float* grad,
   if (units > int64_t(0x7fffffff)) return fail(PXL_E_SIZE, "grid too large for pxl_adam_tv_step's launch");
   hipLaunchKernelGGL(pxl::k_adam_tv, dim3((unsigned)units), dim3(256), 0, (hipStream_t)stream,
                      reinterpret_cast<const float4*>(param_in), reinterpret_cast<float4*>(param_out),
-                     reinterpret_cast<const float4*>(grad), reinterpret_cast<float4*>(exp_avg),
+                     reinterpret_cast<float4*>(grad), reinterpret_cast<float4*>(exp_avg),
                      reinterpret_cast<float4*>(exp_avg_sq), reinterpret_cast<float4*>(grad_abs), Dims{X, Y, Z},
                      tv_sum_sq, tv_weight, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2), bc2_sqrt,
-                     (float)eps, neg_step, half_out, half_dtype);
+                     (float)eps, neg_step, half_out, half_dtype, zero_grad ? 1 : 0);
   return launched("pxl_adam_tv_step");
 }
 

@@ -70,10 +70,11 @@ def adam_step_(param, grad, exp_avg, exp_avg_sq, step, *, lr, betas=(0.9, 0.999)
 
 
 def adam_tv_step_(param_in, param_out, grad, exp_avg, exp_avg_sq, step, *, tv_sumsq, tv_weight, lr,
-                  betas=(0.9, 0.999), eps=1e-8, grad_abs=None, half_out=None):
+                  betas=(0.9, 0.999), eps=1e-8, grad_abs=None, half_out=None, zero_grad=False):
     """tv_backward_(param_in, tv_sumsq, tv_weight, grad) then adam_step_ in one pass
     (pxl_adam_tv_step): the same bits, with the updated parameter written to param_out (a
-    different (X,Y,Z,4) float32 buffer; grad is left unchanged)."""
+    different (X,Y,Z,4) float32 buffer).  grad is left unchanged, or with zero_grad set to zero
+    after use (only its nonzero elements are written), ready for the next step's scatter."""
     if param_in.data_ptr() == param_out.data_ptr():
         raise ValueError("param_out must be a different buffer from param_in")
     X, Y, Z = param_in.shape[:3]
@@ -81,7 +82,8 @@ def adam_tv_step_(param_in, param_out, grad, exp_avg, exp_avg_sq, step, *, tv_su
         _lib.ptr(param_in), _lib.ptr(param_out), _lib.ptr(grad), _lib.ptr(exp_avg), _lib.ptr(exp_avg_sq),
         _lib.ptr(grad_abs), X, Y, Z, _lib.ptr(tv_sumsq), float(tv_weight), float(lr), float(betas[0]),
         float(betas[1]), float(eps), int(step), _lib.ptr(half_out),
-        _HALF[half_out.dtype] if half_out is not None else 0, _lib.stream_handle(param_in.device)))
+        _HALF[half_out.dtype] if half_out is not None else 0, 1 if zero_grad else 0,
+        _lib.stream_handle(param_in.device)))
 
 
 def tv_sumsq(grid, out=None):

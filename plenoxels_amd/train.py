@@ -100,6 +100,7 @@ class Trainer:
         if init_grid is not None:  # the reference starts from zeros (src/grid_functions.py:214)
             self.grid.copy_(init_grid)
         self.grad = torch.zeros_like(self.grid)
+        self._grad_zero = True  # self.grad is all zero (the fused TV + Adam pass can leave it so)
         self.exp_avg = torch.zeros_like(self.grid)
         self.exp_avg_sq = torch.zeros_like(self.grid)
         self.grad_abs = torch.zeros_like(self.grid)
@@ -148,8 +149,9 @@ class Trainer:
             pd = self.pd
             grid = self.grid if self.render_copy is None else self.render_copy
             grad = self.grad
-            if not self.use_blocks:
+            if not self.use_blocks and not self._grad_zero:
                 fill_(grad, 0.0)
+            self._grad_zero = False
         uv = self._uniforms(i).reshape(self.B * self.R, 2).contiguous()
         cam = CameraRays(self.cam_ids, uv, self.c2w, self.theta, self.imgs)
         cfg = MarchConfig(self.S, self.delta, gmin, pd, self.interp, self.near, self.t_stop)
@@ -198,15 +200,19 @@ class Trainer:
             out["betaloss"] = bl
             loss = loss + bl
         if pooled:
-            fill_(self.grad, 0.0)
+            if not self._grad_zero:
+                fill_(self.grad, 0.0)
             pool_backward_(grad, (X, Y, Z), k, max(1, k // 4), self.grad)
+            self._grad_zero = False
         self.step_count += 1
         if fused_tv:
             if self._grid_next is None:
                 self._grid_next = torch.empty_like(self.grid)
+            # the pass leaves the gradient zero for the next step (no separate zero-fill)
             adam_tv_step_(self.grid, self._grid_next, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count,
                           tv_sumsq=self.tvs, tv_weight=self.tv, lr=self.lr, grad_abs=self.grad_abs,
-                          half_out=self.render_copy)
+                          half_out=self.render_copy, zero_grad=True)
+            self._grad_zero = True
             self.grid, self._grid_next = self._grid_next, self.grid
         else:
             adam_step_(self.grid, self.grad, self.exp_avg, self.exp_avg_sq, self.step_count, lr=self.lr,

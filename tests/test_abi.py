@@ -168,12 +168,12 @@ def test_validation_errors_before_any_hip_call():
     assert "occupancy" in _lib.last_error()
     assert L.pxl_adam_tv_step(ctypes.c_void_p(0x100), ctypes.c_void_p(0x100), ctypes.c_void_p(0x200),
                               ctypes.c_void_p(0x300), ctypes.c_void_p(0x400), None, 4, 4, 4,
-                              ctypes.c_void_p(0x500), ctypes.c_float(0.1), 1e-2, 0.9, 0.999, 1e-8, 1, None, 0,
+                              ctypes.c_void_p(0x500), ctypes.c_float(0.1), 1e-2, 0.9, 0.999, 1e-8, 1, None, 0, 0,
                               None) == _lib.PXL_E_INVALID  # param_out == param_in
     assert "param_in" in _lib.last_error()
     assert L.pxl_adam_tv_step(ctypes.c_void_p(0x100), ctypes.c_void_p(0x108), ctypes.c_void_p(0x200),
                               ctypes.c_void_p(0x300), ctypes.c_void_p(0x400), None, 4, 4, 4,
-                              ctypes.c_void_p(0x500), ctypes.c_float(0.1), 1e-2, 0.9, 0.999, 1e-8, 1, None, 0,
+                              ctypes.c_void_p(0x500), ctypes.c_float(0.1), 1e-2, 0.9, 0.999, 1e-8, 1, None, 0, 0,
                               None) == _lib.PXL_E_INVALID  # misaligned param_out
     assert L.pxl_aux_floats(256) == 10
     assert L.pxl_aux_floats(0) == 2

@@ -221,16 +221,25 @@ def test_adam_matches_torch():
 
 
 @pytest.mark.parametrize("half_dtype", [None, torch.bfloat16, torch.float16])
-def test_fused_tv_adam_equals_tv_backward_then_adam(half_dtype):
+@pytest.mark.parametrize("zero_grad", [False, True])
+def test_fused_tv_adam_equals_tv_backward_then_adam(half_dtype, zero_grad):
     """pxl_adam_tv_step: the bits of pxl_tv_bwd into the gradient followed by pxl_adam_step,
     over three steps on an odd-shaped grid (faces, edges and corners of the stencil), with the
-    parameter ping-ponged between two buffers; the gradient buffer is left unchanged."""
+    parameter ping-ponged between two buffers; the gradient buffer is left unchanged, or with
+    zero_grad all zero afterwards, written only where it was nonzero (cells of -0.0 keep
+    their sign)."""
     from plenoxels_amd.optim import adam_step_, adam_tv_step_, tv_backward_, tv_sumsq
     say("fused TV + Adam", half_dtype)
     g = torch.Generator().manual_seed(14)
     dims = (9, 6, 11)
     p0 = torch.randn(*dims, 4, generator=g).to(DEV)
-    grads = [torch.randn(*dims, 4, generator=g).to(DEV) * 10 ** (-i) for i in range(3)]
+    grads = []
+    for i in range(3):  # built on the host, where -0.0 survives the fill, then copied bitwise
+        gr = torch.randn(*dims, 4, generator=g) * 10 ** (-i)
+        gr[:, 2] = 0.0  # cells the scatter never reached
+        gr[0] = -0.0
+        assert bool(torch.signbit(gr[0]).all())
+        grads.append(gr.to(DEV))
     a_p, a_m, a_v, a_ga = p0.clone(), torch.zeros_like(p0), torch.zeros_like(p0), torch.zeros_like(p0)
     b_p, b_alt, b_m, b_v, b_ga = p0.clone(), torch.empty_like(p0), torch.zeros_like(p0), torch.zeros_like(p0), \
         torch.zeros_like(p0)
@@ -247,8 +256,12 @@ def test_fused_tv_adam_equals_tv_backward_then_adam(half_dtype):
         # fused pass
         gb = gr.clone()
         adam_tv_step_(b_p, b_alt, gb, b_m, b_v, t, tv_sumsq=s_t, tv_weight=0.37, lr=0.01, grad_abs=b_ga,
-                      half_out=b_h)
-        assert torch.equal(gb, gr), "the fused step must not write the gradient"
+                      half_out=b_h, zero_grad=zero_grad)
+        if zero_grad:
+            assert int(torch.count_nonzero(gb)) == 0
+            assert bool(torch.signbit(gb[0]).all()), "a zero cell was written"
+        else:
+            assert torch.equal(gb, gr), "the fused step must not write the gradient"
         b_p, b_alt = b_alt, b_p
         for x, y in ((a_p, b_p), (a_m, b_m), (a_v, b_v), (a_ga, b_ga)):
             assert torch.equal(x, y)
