@@ -298,10 +298,12 @@ int pxl_grid_coords(int64_t X, int64_t Y, int64_t Z, float pd, float* coords, in
 int64_t pxl_avg_pool3d_workspace_floats(int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s);
 int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k, int32_t s,
                    float* out, float* workspace, void* stream);
-/* backward: grad_grid (X,Y,Z,4) += sum over windows containing a cell of grad_out / k^3 (the
- * adjoint passes x, y, z); workspace as for the forward. */
+/* backward: grad_grid (X,Y,Z,4) = (accumulate: +=) sum over windows containing a cell of
+ * grad_out / k^3 (the adjoint passes x, y, z; every cell is written, 0 outside all windows);
+ * workspace as for the forward. */
 int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
-                       int32_t s, float* grad_grid, float* workspace, void* stream);
+                       int32_t s, float* grad_grid, int32_t accumulate,
+                       float* workspace, void* stream);
 
 /* scripts/train.py:44-65 (tv_loss): sum_sq (1 double, device, accumulated) += sum over the
  * three axes of squared neighbour differences of grid (X,Y,Z,4). */

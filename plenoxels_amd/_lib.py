@@ -110,7 +110,8 @@ SIGNATURES = {
     "pxl_avg_pool3d": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, c_void_p]),
     "pxl_avg_pool3d_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                          ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, c_void_p]),
+                                          ctypes.c_int32, ctypes.c_int32, c_void_p, ctypes.c_int32, c_void_p,
+                                          c_void_p]),
     "pxl_tv_sumsq": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,
                                     c_void_p]),
     "pxl_tv_bwd": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p,

@@ -998,7 +998,7 @@ int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k
 }
 
 int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, int32_t k,
-                       int32_t s, float* grad_grid, float* workspace, void* stream) {
+                       int32_t s, float* grad_grid, int32_t accumulate, float* workspace, void* stream) {
   int rc;
   if ((rc = pxl_host::check_dims(X, Y, Z)) ||
       (rc = need(grad_out && grad_grid && workspace, "pointers required")) ||
@@ -1013,7 +1013,7 @@ int pxl_avg_pool3d_bwd(const float* grad_out, int64_t X, int64_t Y, int64_t Z, i
   hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * Y * oz, 256)), dim3(256), 0, st,
                      (const float4*)c, X, Y, oz, k, s, oy, 0, d);
   hipLaunchKernelGGL(pxl::k_box_mean_adjoint, dim3(stride_blocks(X * Y * Z, 256)), dim3(256), 0, st,
-                     (const float4*)d, X * Y, Z, (int64_t)1, k, s, oz, 1, (float4*)grad_grid);
+                     (const float4*)d, X * Y, Z, (int64_t)1, k, s, oz, accumulate ? 1 : 0, (float4*)grad_grid);
   return launched("pxl_avg_pool3d_bwd");
 }
 

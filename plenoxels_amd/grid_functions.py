@@ -189,11 +189,13 @@ def _pool_workspace(dims, k, s, device):
     return torch.empty(n, dtype=torch.float32, device=device)
 
 
-def pool_backward_(grad_out, dims, k, s, grad_grid):
-    """grad_grid (X,Y,Z,4) += the average-pooling backward of grad_out (pxl_avg_pool3d_bwd)."""
+def pool_backward_(grad_out, dims, k, s, grad_grid, accumulate=True):
+    """grad_grid (X,Y,Z,4) += (accumulate) or = the average-pooling backward of grad_out
+    (pxl_avg_pool3d_bwd; every cell is written, 0 outside all windows)."""
     go = grad_out.float().contiguous()
     ws = _pool_workspace(dims, k, s, go.device)
-    _lib.check(_lib.lib().pxl_avg_pool3d_bwd(_lib.ptr(go), *dims, k, s, _lib.ptr(grad_grid), _lib.ptr(ws),
+    _lib.check(_lib.lib().pxl_avg_pool3d_bwd(_lib.ptr(go), *dims, k, s, _lib.ptr(grad_grid),
+                                             1 if accumulate else 0, _lib.ptr(ws),
                                              _lib.stream_handle(go.device)))
 
 
@@ -212,8 +214,8 @@ class _PoolFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gout):
-        grad = torch.zeros((*ctx.dims, 4), dtype=torch.float32, device=gout.device)
-        pool_backward_(gout, ctx.dims, ctx.k, ctx.s, grad)
+        grad = torch.empty((*ctx.dims, 4), dtype=torch.float32, device=gout.device)
+        pool_backward_(gout, ctx.dims, ctx.k, ctx.s, grad, accumulate=False)
         return grad, None, None
 
 

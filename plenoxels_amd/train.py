@@ -199,10 +199,8 @@ class Trainer:
             bl = self.beta * (self.bsum.sum() + n_s * beta_term_zero()) / n_s
             out["betaloss"] = bl
             loss = loss + bl
-        if pooled:
-            if not self._grad_zero:
-                fill_(self.grad, 0.0)
-            pool_backward_(grad, (X, Y, Z), k, max(1, k // 4), self.grad)
+        if pooled:  # the full grid's gradient, every cell written
+            pool_backward_(grad, (X, Y, Z), k, max(1, k // 4), self.grad, accumulate=False)
             self._grad_zero = False
         self.step_count += 1
         if fused_tv:

@@ -170,6 +170,27 @@ def test_average_pool3d_grid_fwd_bwd(k, s):
     assert ok, info
 
 
+def test_pool_backward_overwrite_and_accumulate():
+    """pxl_avg_pool3d_bwd writes every cell: overwriting a NaN-filled buffer gives the bits of
+    accumulating into zeros (cells outside every window: 0, as the plane x = 39 of 40 is with
+    k = 9, s = 2), and accumulating onto a filled buffer adds to it."""
+    say("pool backward modes")
+    g = torch.Generator().manual_seed(21)
+    dims, k, s = (40, 37, 35), 9, 2
+    shape = tuple((d - k) // s + 1 for d in dims) + (4,)
+    gout = torch.randn(shape, generator=g).to(DEV)
+    base = torch.randn(*dims, 4, generator=g).to(DEV)
+    z = torch.zeros(*dims, 4, device=DEV)
+    gf.pool_backward_(gout, dims, k, s, z)
+    o = torch.full((*dims, 4), float("nan"), device=DEV)
+    gf.pool_backward_(gout, dims, k, s, o, accumulate=False)
+    assert torch.equal(o, z)
+    a = base.clone()
+    gf.pool_backward_(gout, dims, k, s, a)
+    assert torch.equal(a, base + z)
+    assert float(z[39].abs().max()) == 0.0  # windows along x cover 0..38
+
+
 def test_generate_grid_bitwise():
     say("generate_grid")
     for dims, pd in (((5, 6, 7), 0.25), ((64, 63, 2), 3 / 63)):
