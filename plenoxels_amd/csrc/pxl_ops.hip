@@ -390,9 +390,11 @@ __global__ __launch_bounds__(256) void k_box_mean(const float4* __restrict__ in,
   }
 }
 
-// The z pass (inner = 1): one workgroup per row loads the row into LDS with coalesced reads,
-// then each thread sums its windows from LDS (threads of the generic pass would each read a
-// separate memory segment, their windows being s cells apart).
+// The z pass (inner = 1): a workgroup loads as many whole rows as fit in LDS (four of 512
+// cells) with coalesced reads, then each thread sums a (row, window) pair's cells from LDS in
+// order (threads of the generic pass would each read a separate memory segment, their
+// windows being s cells apart); with several rows per batch every thread has a window even
+// when a row has few (19 at k = 93 over 512 cells).
 constexpr int kRowLdsCells = 2048;  // rows up to 2048 cells (32 KiB of LDS)
 
 __global__ __launch_bounds__(256) void k_box_mean_rows(const float4* __restrict__ in, int64_t rows,
@@ -400,19 +402,23 @@ __global__ __launch_bounds__(256) void k_box_mean_rows(const float4* __restrict_
                                                        float4* __restrict__ out) {
   __shared__ float4 s_row[kRowLdsCells];
   const float fk = (float)k;
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_row[i] = in[r * n + i];
+  const int per = kRowLdsCells / n;  // rows per batch (n <= kRowLdsCells)
+  for (int64_t r0 = (int64_t)blockIdx.x * per; r0 < rows; r0 += (int64_t)gridDim.x * per) {
+    const int nr = (int)min((int64_t)per, rows - r0);
+    for (int i = threadIdx.x; i < nr * n; i += blockDim.x) s_row[i] = in[r0 * n + i];
     __syncthreads();
-    for (int o = threadIdx.x; o < n_out; o += blockDim.x) {
+    for (int t = threadIdx.x; t < nr * n_out; t += blockDim.x) {
+      const int rr = t / n_out, o = t - rr * n_out;
+      const float4* src = s_row + rr * n + o * s;
       float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
       for (int d = 0; d < k; ++d) {
-        const float4 v = s_row[o * s + d];
+        const float4 v = src[d];
         acc.x += v.x;
         acc.y += v.y;
         acc.z += v.z;
         acc.w += v.w;
       }
-      out[r * n_out + o] = make_float4(acc.x / fk, acc.y / fk, acc.z / fk, acc.w / fk);
+      out[(r0 + rr) * n_out + o] = make_float4(acc.x / fk, acc.y / fk, acc.z / fk, acc.w / fk);
     }
     __syncthreads();
   }
@@ -985,7 +991,9 @@ int pxl_avg_pool3d(const float* grid, int64_t X, int64_t Y, int64_t Z, int32_t k
   float4* b = a + X * Y * oz;                         // (X, oy, oz)
   hipStream_t st = (hipStream_t)stream;
   if (Z <= pxl::kRowLdsCells)
-    hipLaunchKernelGGL(pxl::k_box_mean_rows, dim3((unsigned)std::min<int64_t>(X * Y, int64_t(1) << 20)),
+    hipLaunchKernelGGL(pxl::k_box_mean_rows,
+                       dim3((unsigned)std::min<int64_t>((X * Y + pxl::kRowLdsCells / Z - 1) / (pxl::kRowLdsCells / Z),
+                                                        int64_t(1) << 20)),
                        dim3(256), 0, st, (const float4*)grid, X * Y, (int)Z, k, s, (int)oz, a);
   else
     hipLaunchKernelGGL(pxl::k_box_mean, dim3(stride_blocks(X * Y * oz, 256)), dim3(256), 0, st,
