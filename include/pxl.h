@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PXL_ABI_VERSION 1
+#define PXL_ABI_VERSION 2
 
 /* return codes */
 #define PXL_OK 0

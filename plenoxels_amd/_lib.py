@@ -20,6 +20,7 @@ PXL_E_HIP = -4
 PXL_E_NO_DEVICE = -5
 
 DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
+ABI_VERSION = 2  # PXL_ABI_VERSION of the include/pxl.h these signatures mirror
 INTERP_NEAREST, INTERP_TRILINEAR = 0, 1
 FLAG_EXACT_DIV = 1
 
@@ -151,8 +152,8 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        if L.pxl_abi_version() != 1:
-            raise ImportError("libpxl ABI version mismatch")
+        if L.pxl_abi_version() != ABI_VERSION:
+            raise ImportError(f"libpxl ABI version {L.pxl_abi_version()}, these bindings expect {ABI_VERSION}")
         _lib = L
     return _lib
 

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
         assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
     assert set(_lib.SIGNATURES) == set(names)
-    assert L.pxl_abi_version() == 1
+    assert L.pxl_abi_version() == _lib.ABI_VERSION == 2
 
 
 def test_struct_layout_matches_header(tmp_path):
