@@ -110,7 +110,6 @@ static void eval_sample(const ref_params* p, const double* grid, const uint32_t*
     cs[a][1] = wrap((int64_t)floorf(n[a]), dims[a]);
     fr[a] = (double)(n[a] - truncf(n[a]));
   }
-  int64_t floor_cell = (cs[0][1] * p->Y + cs[1][1]) * p->Z + cs[2][1];
   if (!occ_bit(p, occ, cs[0][1], cs[1][1], cs[2][1])) return;
   s->keep = 1;
   s->ncorner = 8;
