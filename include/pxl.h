@@ -13,6 +13,9 @@
  *   - Work is enqueued asynchronously on `stream` (a hipStream_t; NULL = default stream).
  *   - Return 0 on success or a negative PXL_E_* code; pxl_last_error() then describes it
  *     (thread-local).  Argument validation happens before any HIP call.
+ *   - A call over zero rays / samples / elements succeeds and launches nothing, whatever the
+ *     pointers of its per-ray or per-element buffers (torch gives an empty tensor a NULL data
+ *     pointer); descriptors, the grid and grid-shaped buffers are still checked.
  *   - Grid layout (src/grid_functions.py:214-215): row-major (X,Y,Z,4), channel innermost,
  *     channels [R,G,B,alpha]; linear cell index L = (i*Y + j)*Z + k.  Gradients are always
  *     FP32 (X,Y,Z,4) and are ACCUMULATED into (the caller zero-fills when it wants d/dgrid).

@@ -790,7 +790,7 @@ int pxl_sample_points(const float* origins, int64_t rays_per_origin, const float
                       int64_t n_rays, int32_t n_samples, float delta, float near, float* out,
                       void* stream) {
   int rc;
-  if ((rc = need(origins && dirs && out, "origins, dirs and out are required")) ||
+  if ((rc = need(n_rays == 0 || n_samples == 0 || (origins && dirs && out), "origins, dirs and out are required")) ||
       (rc = need(n_rays >= 0 && n_samples >= 0 && rays_per_origin > 0, "negative size")))
     return rc;
   if (n_rays == 0 || n_samples == 0) return PXL_OK;
@@ -812,7 +812,7 @@ int pxl_coord_min(const float* coords, int64_t n, float* out3, void* stream) {
 int pxl_normalize_points(const float* p, int64_t n, const float* min3, float pd, float inv_pd,
                          int32_t flags, float* out, void* stream) {
   int rc;
-  if ((rc = need(p && min3 && out && n >= 0, "p, min3, out required and n >= 0")) ||
+  if ((rc = need(n >= 0 && (n == 0 || (p && min3 && out)), "p, min3, out required and n >= 0")) ||
       (rc = need((flags & ~PXL_FLAG_EXACT_DIV) == 0, "unknown flags")))
     return rc;
   if (n == 0) return PXL_OK;
@@ -825,7 +825,7 @@ int pxl_in_bounds(const void* idx, int32_t is_float, int64_t n, const int64_t di
                   uint8_t* mask, void* stream) {
   int rc;
   Dims d;
-  if ((rc = need(idx && mask && n >= 0, "idx, mask required and n >= 0")) || (rc = dims3(dims, d)))
+  if ((rc = need(n >= 0 && (n == 0 || (idx && mask)), "idx, mask required and n >= 0")) || (rc = dims3(dims, d)))
     return rc;
   if (n == 0) return PXL_OK;
   if (is_float)
@@ -840,7 +840,7 @@ int pxl_in_bounds(const void* idx, int32_t is_float, int64_t n, const int64_t di
 int pxl_wrap_indices(int64_t* idx, int64_t n, const int64_t dims[3], void* stream) {
   int rc;
   Dims d;
-  if ((rc = need(idx && n >= 0, "idx required and n >= 0")) || (rc = dims3(dims, d))) return rc;
+  if ((rc = need(n >= 0 && (n == 0 || idx), "idx required and n >= 0")) || (rc = dims3(dims, d))) return rc;
   if (n == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_wrap, dim3(stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, idx,
                      n, d);
@@ -851,7 +851,7 @@ int pxl_nearest_gather(const float* n, int64_t N, const pxl_grid* grid, float* v
                        uint8_t* mask, void* stream) {
   int rc;
   Dims d;
-  if ((rc = f32_grid(grid, d)) || (rc = need(n && vals && N >= 0, "n, vals required and N >= 0")))
+  if ((rc = f32_grid(grid, d)) || (rc = need(N >= 0 && (N == 0 || (n && vals)), "n, vals required and N >= 0")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_nearest_gather, dim3(stride_blocks(N, 256)), dim3(256), 0,
@@ -864,7 +864,7 @@ int pxl_nearest_scatter(const float* n, int64_t N, const pxl_grid* grid, const f
   int rc;
   Dims d;
   if ((rc = f32_grid(grid, d)) ||
-      (rc = need(n && grad_vals && grad_grid && N >= 0, "n, grad_vals, grad_grid required")))
+      (rc = need(N >= 0 && (N == 0 || (n && grad_vals && grad_grid)), "n, grad_vals, grad_grid required")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_nearest_scatter, dim3(stride_blocks(N, 256)), dim3(256), 0,
@@ -874,7 +874,7 @@ int pxl_nearest_scatter(const float* n, int64_t N, const pxl_grid* grid, const f
 
 int pxl_corner_indices(const float* n, int64_t N, int64_t* corners, void* stream) {
   int rc;
-  if ((rc = need(n && corners && N >= 0, "n, corners required and N >= 0"))) return rc;
+  if ((rc = need(N >= 0 && (N == 0 || (n && corners)), "n, corners required and N >= 0"))) return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_corner_indices, dim3(stride_blocks(N, 256)), dim3(256), 0,
                      (hipStream_t)stream, n, N, corners);
@@ -885,7 +885,7 @@ int pxl_collect_cells(const float* n, int64_t N, const pxl_grid* grid, float* ou
                       void* stream) {
   int rc;
   Dims d;
-  if ((rc = f32_grid(grid, d)) || (rc = need(n && out && N >= 0, "n, out required and N >= 0")))
+  if ((rc = f32_grid(grid, d)) || (rc = need(N >= 0 && (N == 0 || (n && out)), "n, out required and N >= 0")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_collect, dim3(stride_blocks(N, 256)), dim3(256), 0, (hipStream_t)stream,
@@ -898,7 +898,7 @@ int pxl_collect_cells_bwd(const float* n, int64_t N, const pxl_grid* grid, const
   int rc;
   Dims d;
   if ((rc = f32_grid(grid, d)) ||
-      (rc = need(n && grad_out && grad_grid && N >= 0, "n, grad_out, grad_grid required")))
+      (rc = need(N >= 0 && (N == 0 || (n && grad_out && grad_grid)), "n, grad_out, grad_grid required")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_collect_bwd, dim3(stride_blocks(N, 256)), dim3(256), 0,
@@ -911,7 +911,7 @@ int pxl_trilinear(const float* n, const int64_t* corners, int64_t N, const pxl_g
   int rc;
   Dims d;
   if ((rc = f32_grid(grid, d)) ||
-      (rc = need(n && corners && out && N >= 0, "n, corners, out required and N >= 0")))
+      (rc = need(N >= 0 && (N == 0 || (n && corners && out)), "n, corners, out required and N >= 0")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_trilinear, dim3(stride_blocks(N, 256)), dim3(256), 0, (hipStream_t)stream,
@@ -924,7 +924,7 @@ int pxl_trilinear_bwd(const float* n, const int64_t* corners, int64_t N, const p
   int rc;
   Dims d;
   if ((rc = f32_grid(grid, d)) ||
-      (rc = need(n && corners && grad_out && grad_grid && N >= 0, "pointers required")))
+      (rc = need(N >= 0 && (N == 0 || (n && corners && grad_out && grad_grid)), "pointers required")))
     return rc;
   if (N == 0) return PXL_OK;
   hipLaunchKernelGGL(pxl::k_trilinear_bwd, dim3(stride_blocks(N, 256)), dim3(256), 0,
@@ -934,7 +934,7 @@ int pxl_trilinear_bwd(const float* n, const int64_t* corners, int64_t N, const p
 
 int pxl_composite_fwd(const float* samples, int64_t R, int32_t S, float* out, void* stream) {
   int rc;
-  if ((rc = need(samples && out && R >= 0 && S >= 0, "samples, out required; sizes >= 0")))
+  if ((rc = need(R >= 0 && S >= 0 && (R == 0 || ((samples || S == 0) && out)), "samples, out required; sizes >= 0")))
     return rc;
   if (R == 0) return PXL_OK;
   if (S > 64 * pxl::kMaxCompositeChunks) return fail(PXL_E_SIZE, "S > 65536");
@@ -950,7 +950,7 @@ int pxl_composite_fwd(const float* samples, int64_t R, int32_t S, float* out, vo
 int pxl_composite_bwd(const float* samples, int64_t R, int32_t S, const float* grad_out,
                       float* grad_samples, void* stream) {
   int rc;
-  if ((rc = need(samples && grad_out && grad_samples && R >= 0 && S >= 0, "pointers required")))
+  if ((rc = need(R >= 0 && S >= 0 && (R == 0 || S == 0 || (samples && grad_out && grad_samples)), "pointers required")))
     return rc;
   if (R == 0 || S == 0) return PXL_OK;
   if (S > 64 * pxl::kMaxCompositeChunks) return fail(PXL_E_SIZE, "S > 65536");
@@ -1102,7 +1102,7 @@ int pxl_adam_step(float* param, const float* grad, float* exp_avg, float* exp_av
                   float* grad_abs, int64_t n, double lr, double beta1, double beta2, double eps,
                   int64_t step, void* half_out, int32_t half_dtype, void* stream) {
   int rc;
-  if ((rc = need(param && grad && exp_avg && exp_avg_sq && n >= 0, "param, grad, moments required")) ||
+  if ((rc = need(n >= 0 && (n == 0 || (param && grad && exp_avg && exp_avg_sq)), "param, grad, moments required")) ||
       (rc = need(step >= 1, "step counts from 1")) ||
       (rc = need(!half_out || half_dtype == PXL_DTYPE_BF16 || half_dtype == PXL_DTYPE_F16,
                  "half_dtype must be PXL_DTYPE_BF16 or PXL_DTYPE_F16")))

@@ -925,6 +925,7 @@ int check_march(const pxl_march* m) {
 int check_rays(const pxl_rays* r, bool need_imgs) {
   if (!r) return fail(PXL_E_INVALID, "rays descriptor is NULL");
   if (r->n_rays < 0) return fail(PXL_E_INVALID, "n_rays is negative");
+  if (r->n_rays == 0) return PXL_OK;  // an empty batch reads nothing (torch gives it NULL buffers)
   if (r->origins) {
     if (!r->dirs) return fail(PXL_E_INVALID, "origins given without dirs");
   } else {
@@ -1042,8 +1043,8 @@ int pxl_render_fwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
                    float* out_rgba, float* aux, void* stream) {
   int rc;
   if ((rc = check_grid(grid)) || (rc = check_march(march)) || (rc = check_rays(rays, false))) return rc;
-  if (!out_rgba) return fail(PXL_E_INVALID, "out_rgba is NULL");
   if (rays->n_rays == 0) return PXL_OK;
+  if (!out_rgba) return fail(PXL_E_INVALID, "out_rgba is NULL");
   pxl::KParams P = make_params(grid, march, rays);
   hipStream_t s = (hipStream_t)stream;
   PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
@@ -1059,8 +1060,9 @@ int pxl_render_bwd(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
                    const float* grad_rgba, const float* aux, float* grad_grid, void* stream) {
   int rc;
   if ((rc = check_grid(grid)) || (rc = check_march(march)) || (rc = check_rays(rays, false))) return rc;
+  if (rays->n_rays == 0) return PXL_OK;
   if (!grad_rgba || !aux || !grad_grid) return fail(PXL_E_INVALID, "grad_rgba, aux and grad_grid are required");
-  if (rays->n_rays == 0 || march->n_samples == 0) return PXL_OK;
+  if (march->n_samples == 0) return PXL_OK;
   pxl::KParams P = make_params(grid, march, rays);
   hipStream_t s = (hipStream_t)stream;
   PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
@@ -1182,9 +1184,9 @@ int pxl_camera_rays(const pxl_rays* rays, float* origins, float* dirs, float* ta
                     void* stream) {
   int rc;
   if ((rc = check_rays(rays, targets != nullptr))) return rc;
+  if (rays->n_rays == 0) return PXL_OK;
   if (rays->origins) return fail(PXL_E_INVALID, "pxl_camera_rays needs camera-form rays");
   if (!origins || !dirs) return fail(PXL_E_INVALID, "origins and dirs outputs are required");
-  if (rays->n_rays == 0) return PXL_OK;
   pxl::KParams P = make_params(nullptr, nullptr, rays);
   hipStream_t s = (hipStream_t)stream;
   return for_ray_pieces(P, 256, [&](const pxl::KParams& Q, unsigned nb) {

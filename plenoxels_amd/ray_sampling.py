@@ -109,8 +109,8 @@ class _CompositeFn(torch.autograd.Function):
         shape = samples.shape
         S = shape[-2]
         x = samples.float().contiguous()
-        R = x.numel() // (4 * S) if S else 0
         out = torch.empty(shape[:-2] + (4,), dtype=torch.float32, device=x.device)
+        R = out.numel() // 4  # S = 0 composites to zeros, as the reference's empty sums
         _lib.check(_lib.lib().pxl_composite_fwd(_lib.ptr(x), R, S, _lib.ptr(out),
                                                 _lib.stream_handle(x.device)))
         ctx.save_for_backward(x)

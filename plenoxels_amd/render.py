@@ -326,8 +326,9 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
     n_samples_total = (R * cfg.num_samples) if n_total_samples is None else int(n_total_samples)
     loss = _lib.Loss()
     loss.targets = None if targets is None else targets.data_ptr()
-    loss.grad_scale = 2.0 / (4.0 * n_rays_total)
-    loss.beta_scale = float(beta) / n_samples_total if beta else 0.0
+    # an empty batch (R = 0) launches nothing; its scales are never used
+    loss.grad_scale = 2.0 / (4.0 * n_rays_total) if n_rays_total > 0 else 0.0
+    loss.beta_scale = float(beta) / n_samples_total if beta and n_samples_total > 0 else 0.0
     loss.ray_sq_err = None if ray_sq_err is None else ray_sq_err.data_ptr()
     loss.ray_beta = None if ray_beta is None else ray_beta.data_ptr()
     g = _grid_struct(grid, occupancy)

@@ -117,9 +117,11 @@ def test_validation_errors_before_any_hip_call():
     r2.n_rays = -3
     assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
     g2, m2, r2 = _valid()
+    r2.n_rays = 4
     r2.origins = None  # camera form without cameras
     assert _fwd(g2, m2, r2) == _lib.PXL_E_INVALID
     g2, m2, r2 = _valid()
+    r2.n_rays = 4
     assert _fwd(g2, m2, r2, out=0) == _lib.PXL_E_INVALID
     g2, m2, r2 = _valid()
     m2.n_samples = 600
@@ -178,6 +180,36 @@ def test_validation_errors_before_any_hip_call():
     assert L.pxl_aux_floats(256) == 10
     assert L.pxl_aux_floats(0) == 2
     assert L.pxl_aux_floats(65) == 6
+
+
+def test_empty_inputs_are_no_ops_whatever_their_pointers():
+    """torch hands an empty tensor a NULL data pointer: every zero-size call succeeds without
+    reading its buffers (and launches nothing), while negative sizes stay errors."""
+    L = _lib.lib()
+    g, m, r = _valid()
+    r.origins = r.dirs = None
+    assert _fwd(g, m, r, out=0) == _lib.PXL_OK
+    assert L.pxl_render_bwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), None, None, None,
+                            None) == _lib.PXL_OK
+    loss = _lib.Loss()
+    loss.grad_scale = 0.0
+    assert L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
+                            None, ctypes.c_void_p(0x5000), None) == _lib.PXL_OK
+    assert L.pxl_train_step(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), ctypes.byref(loss),
+                            None, None, None) == _lib.PXL_E_INVALID  # the gradient grid is never empty
+    assert L.pxl_camera_rays(ctypes.byref(r), None, None, None, None) == _lib.PXL_OK
+    assert L.pxl_sample_points(None, 1, None, 0, 16, ctypes.c_float(0.1), ctypes.c_float(0.0), None,
+                               None) == _lib.PXL_OK
+    assert L.pxl_composite_fwd(None, 0, 16, None, None) == _lib.PXL_OK
+    assert L.pxl_composite_bwd(None, 0, 16, None, None, None) == _lib.PXL_OK
+    assert L.pxl_composite_fwd(None, -1, 16, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_nearest_gather(None, 0, ctypes.byref(g), None, None, None) == _lib.PXL_OK
+    assert L.pxl_nearest_gather(None, -1, ctypes.byref(g), None, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_trilinear_bwd(None, None, 0, ctypes.byref(g), None, None, None) == _lib.PXL_OK
+    assert L.pxl_corner_indices(None, 0, None, None) == _lib.PXL_OK
+    assert L.pxl_adam_step(None, None, None, None, None, 0, 1e-3, 0.9, 0.999, 1e-8, 1, None, 0,
+                           None) == _lib.PXL_OK
+    assert L.pxl_fill_f32(None, 0, ctypes.c_float(0.0), None) == _lib.PXL_OK
 
 
 def test_corner_block_validation_before_any_hip_call():

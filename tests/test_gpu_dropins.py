@@ -151,6 +151,24 @@ def test_compute_alpha_weighted_pixels_fwd_bwd(S):
     assert ok, info
 
 
+def test_compute_alpha_weighted_pixels_empty_batch():
+    say("compute_alpha_weighted_pixels on zero rays")
+    xd = torch.rand(0, 16, 4, device=DEV).requires_grad_(True)
+    out = rs.compute_alpha_weighted_pixels(xd)
+    want = plain.compute_alpha_weighted_pixels(torch.rand(0, 16, 4))
+    assert out.shape == want.shape == (0, 4)
+    out.sum().backward()
+    assert xd.grad is not None and xd.grad.shape == (0, 16, 4)
+    # zero samples per ray: the reference's sums over an empty sample axis are zero
+    xz = torch.rand(3, 0, 4, device=DEV).requires_grad_(True)
+    out = rs.compute_alpha_weighted_pixels(xz)
+    want = plain.compute_alpha_weighted_pixels(torch.rand(3, 0, 4))
+    assert out.shape == want.shape == (3, 4)
+    assert torch.equal(out.detach().cpu(), want)
+    out.sum().backward()
+    assert xz.grad is not None and xz.grad.shape == (3, 0, 4)
+
+
 @pytest.mark.parametrize("k,s", [(3, 1), (5, 2), (13, 3), (33, 8)])
 def test_average_pool3d_grid_fwd_bwd(k, s):
     say("average_pool3d_grid", k, s)

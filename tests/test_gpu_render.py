@@ -300,3 +300,57 @@ def test_nan_cell_propagates_like_torch_clamp(interp):
     assert bool((nan_got <= torch.isnan(want).any(1)).all())
     ok = ~torch.isnan(want).any(1)
     assert (got[ok] - want[ok]).abs().max().item() <= 1e-5
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_empty_ragged_and_missing_batches(interp):
+    """Batches of 0, 1, 63, 65 and 130 rays (the wave is 64 lanes) and a batch whose rays all
+    miss the grid, through render_rays (autograd), train_step and, for trilinear, the corner
+    blocks: RGBA and gradients vs the plain chain; an empty batch changes nothing."""
+    from plenoxels_amd import corner_blocks
+    say("empty / ragged / missing batches", interp)
+    grid, o, d, gmin, pd = small_scene(30, R=130)
+    tgt = torch.rand(130, 4, generator=torch.Generator().manual_seed(31))
+    kw = dict(num_samples=96, delta_step=0.04, points_distance=pd, interp=interp)
+    cfg = MarchConfig(96, 0.04, tuple(gmin.tolist()), pd, interp)
+    gd = grid.to(DEV)
+    cases = [(R, o[:R], d[:R], tgt[:R]) for R in (0, 1, 63, 65, 130)]
+    cases.append(("miss", o[:70], -d[:70], tgt[:70]))  # from below the grid, pointing away
+    for name, oc, dc, tc in cases:
+        say("  batch", name)
+        R = oc.shape[0]
+        od, dd, td = oc.contiguous().to(DEV), dc.contiguous().to(DEV), tc.contiguous().to(DEV)
+        gr = gd.clone().requires_grad_(True)
+        out = render_rays(gr, od, dd, grid_min=gmin.tolist(), **kw)
+        assert tuple(out.shape) == (R, 4)
+        (out * td).sum().backward()
+        gstep = torch.full(grid.shape, 7.0, device=DEV) if R == 0 else torch.zeros(grid.shape, device=DEV)
+        sout = torch.empty(R, 4, device=DEV)
+        sq = torch.empty(R, device=DEV)
+        train_step(gd, gstep, cfg=cfg, origins=od, dirs=dd, targets=td, out=sout, ray_sq_err=sq)
+        if R == 0:
+            assert int(torch.count_nonzero(gr.grad)) == 0
+            assert bool((gstep == 7.0).all()), "an empty batch wrote the gradient"
+            continue
+        want, gwant = _oracle(grid, oc, dc, tc, gmin=gmin, div_mode="reciprocal", **kw)
+        assert (out.detach().cpu() - want).abs().max().item() <= 1e-5
+        assert (sout.cpu() - want).abs().max().item() <= 1e-5
+        ok, info = grad_close(gstep.cpu(), gwant)
+        assert ok, info
+        gp = grid.clone().requires_grad_(True)
+        (plain.render(gp, oc, dc, gmin=gmin, div_mode="reciprocal", **kw) * tc).sum().backward()
+        ok, info = grad_close(gr.grad.cpu(), gp.grad)
+        assert ok, info
+        if name == "miss":
+            assert want.abs().max().item() == 0.0 and int(torch.count_nonzero(gwant)) == 0
+            assert int(torch.count_nonzero(sout)) == 0 and int(torch.count_nonzero(gr.grad)) == 0
+        if interp == "trilinear":
+            blocks = corner_blocks(gd)
+            gb = torch.full(grid.shape, 7.0, device=DEV)
+            bout = torch.empty(R, 4, device=DEV)
+            train_step(gd, gb, cfg=cfg, origins=od, dirs=dd, targets=td, out=bout, blocks=blocks,
+                       overwrite=True)
+            assert torch.equal(bout, sout)
+            ok, info = grad_close(gb.cpu(), gwant)
+            assert ok, info
+            assert int(torch.count_nonzero(blocks)) == 0
