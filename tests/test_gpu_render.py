@@ -206,6 +206,33 @@ def test_long_rays_and_saturation_vs_fp64():
     assert ok
 
 
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_transmittance_scan_fp64_on_gpu_near_saturation(interp):
+    """The transmittance scan of long rays (S = 600) near opacity saturation, checked in FP64
+    on the GPU: the reference chain's FP32 sample values (positions, lookup, clamp; the
+    oracle's torch ops run on the device) composited in float64 on the device, against the
+    fused FP32 kernel's RGBA and its transmittance 1 - A.  The only FP64 arithmetic on the GPU."""
+    say("FP64 transmittance scan on the GPU", interp)
+    grid, o, d, gmin, pd = small_scene(70, X=32, R=512, pd=0.08, alpha_scale=0.9, p_opaque=0.05)
+    S, delta = 600, 0.0075
+    kw = dict(num_samples=S, delta_step=delta, points_distance=pd, interp=interp)
+    gdev, od, dd = grid.to(DEV), o.to(DEV), d.to(DEV)
+    out = render_rays(gdev, od, dd, grid_min=gmin.tolist(), **kw).detach()
+    with torch.no_grad():
+        n = plain.normalize_samples(plain.sample_positions(od, dd, S, delta), gmin.to(DEV), pd, "reciprocal")
+        lookup = plain.nearest_values if interp == "nearest" else plain.trilinear_values
+        vals64 = lookup(n, gdev)[0].reshape(-1, S, 4).double()
+        want = plain.compute_alpha_weighted_pixels(vals64)
+        T64 = (1 - vals64[..., 3]).prod(-1)
+    sat = (T64 < 1e-4).double().mean().item()
+    say("  rays with final transmittance < 1e-4:", sat)
+    assert sat > 0.3
+    err = (out.double() - want).abs().max().item()
+    terr = ((1 - out[:, 3].double()) - T64).abs().max().item()
+    say("  rgba max abs err", err, "transmittance max abs err", terr)
+    assert err <= 1e-5 and terr <= 1e-5
+
+
 def test_camera_rays_bitwise_vs_plain():
     say("camera rays")
     B, H, W, R = 5, 37, 37, 4096
