@@ -19,7 +19,8 @@ import torch
 
 from oracle import cref
 from oracle import plain
-from plenoxels_amd import CameraRays, MarchConfig, camera_rays, occupancy_dilate, pack_occupancy, scenes, train_step
+from plenoxels_amd import (CameraRays, LossMean, MarchConfig, camera_rays, occupancy_dilate, pack_occupancy, scenes,
+                           train_step)
 from util import grad_close, say
 
 pytestmark = pytest.mark.gpu
@@ -141,20 +142,26 @@ def test_sparse256_half_storage_error_report(dtype):
     _, g64, _ = cref.render(w.grid.numpy(), o.numpy(), d.numpy(), occupancy_bits=bits_np, grad_rgba=up, **kw)
     bits = occupancy_dilate(pack_occupancy(w.occupancy.to(DEV)), w.dims)
     cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance, "trilinear", w.near)
+    loss64 = float(np.mean((rgba64 - tgt.double().numpy()) ** 2))
+    loss_mean = LossMean(DEV)
     res = {}
     for name, dt in (("f32", torch.float32), ("half", dtype)):
         gg = torch.zeros(w.grid.shape, device=DEV)
         out = torch.empty(n, 4, device=DEV)
+        sq = torch.empty(n, device=DEV)
         train_step(w.grid.to(dt).to(DEV), gg, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV),
-                   out=out, occupancy=bits)
+                   out=out, occupancy=bits, ray_sq_err=sq)
+        loss = float(loss_mean(sq))
         e = np.abs(out.cpu().double().numpy() - rgba64)
         gk = gg.cpu().double().numpy()
         gerr = np.abs(gk - g64).max() / np.abs(g64).max()
         psnr = 10 * np.log10(1.0 / np.mean((out.cpu().double().numpy() - tgt.double().numpy()) ** 2))
         res[name] = {"rgba_max_abs_err": float(e.max()), "rgba_mean_abs_err": float(e.mean()),
-                     "grad_rel_err": float(gerr), "psnr_vs_targets_db": float(psnr)}
+                     "grad_rel_err": float(gerr), "psnr_vs_targets_db": float(psnr),
+                     "loss": loss, "loss_rel_err_vs_fp64": abs(loss - loss64) / loss64}
     psnr64 = 10 * np.log10(1.0 / np.mean((rgba64 - tgt.double().numpy()) ** 2))
     res["fp64_psnr_vs_targets_db"] = float(psnr64)
+    res["fp64_loss"] = loss64
     res["psnr_delta_half_vs_f32_db"] = res["f32"]["psnr_vs_targets_db"] - res["half"]["psnr_vs_targets_db"]
     say("  ", json.dumps(res))
     os.makedirs(OUT, exist_ok=True)
@@ -162,4 +169,5 @@ def test_sparse256_half_storage_error_report(dtype):
         json.dump(res, f, indent=1)
     assert res["f32"]["rgba_max_abs_err"] <= 1e-5
     assert res["f32"]["grad_rel_err"] <= 1e-4
+    assert res["f32"]["loss_rel_err_vs_fp64"] <= 1e-5
     assert abs(res["psnr_delta_half_vs_f32_db"]) <= 0.1
