@@ -61,7 +61,7 @@ def parse():
                     help="trilinear gradient path: corner blocks + reduce, or atomics into the grid")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--plain-chunk", type=int, default=32768)
-    ap.add_argument("--cpu-rays", type=int, default=2048)
+    ap.add_argument("--cpu-rays", type=int, default=16384)
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-accounting", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -251,13 +251,15 @@ def cpu_baseline(S, args):
                                       t_stop=w.t_stop)
     run()
     reps = 3
-    t0 = time.perf_counter()
+    times = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         run()
-    dt = (time.perf_counter() - t0) / reps
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return {"value": n / dt, "unit": "rays/s", "cores": cores, "kind": "port",
             "sample": f"first {n} rays of the {S.R}-ray sparse256 batch ({w.interp}, S={w.num_samples}, "
-                      f"fwd+bwd, plain PyTorch chain on CPU, mean of {reps} after 1 warm-up)"}
+                      f"fwd+bwd, plain PyTorch chain on CPU, median of {reps} after 1 warm-up)"}
 
 
 def load_traffic(args):
