@@ -272,3 +272,14 @@ def test_python_wrappers_reject_cpu_tensors():
     o = torch.zeros(2, 3)
     with pytest.raises(ValueError):
         render_rays(grid, o, o, num_samples=4, delta_step=0.1, grid_min=(0, 0, 0), points_distance=0.1)
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """No CPU fallback: without libpxl.so every product entry raises instead of computing."""
+    from plenoxels_amd import LossMean
+    monkeypatch.setattr(_lib, "LIB_PATH", os.path.join(ROOT, "no-such-dir", "libpxl.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(ImportError, match="not built"):
+        _lib.lib()
+    with pytest.raises(ImportError, match="not built"):
+        LossMean("cpu")
