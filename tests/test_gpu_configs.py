@@ -126,6 +126,39 @@ def test_view800_gradient_vs_autograd_and_fp64_finite_differences():
     assert k_vs_fd <= 1e-4
 
 
+def test_sparse256_full_batch_vs_plain_gpu():
+    """The benchmark's whole batch (256^3, ~5% occupancy, 2^18 rays, 256 trilinear samples,
+    termination at 1e-4) through the corner-block step, against the plain chain's autograd on
+    the same GPU and the same rays."""
+    say("sparse256 full batch vs plain chain on the GPU")
+    from plenoxels_amd import corner_blocks
+    w = scenes.make_workload("sparse256")
+    o, d = _rays(w)
+    R = o.shape[0]
+    assert R == 1 << 18
+    tgt = scenes.random_targets(R).to(DEV)
+    od, dd = o.to(DEV), d.to(DEV)
+    occ_dev = w.occupancy.to(DEV)
+    g = w.grid.to(DEV).requires_grad_(True)
+    want, _ = plain.render_chunked_backward(g, od, dd, tgt, chunk_rays=32768, num_samples=w.num_samples,
+                                            delta_step=w.delta_step, gmin=torch.tensor(w.grid_min, device=DEV),
+                                            points_distance=w.points_distance, interp="trilinear", near=w.near,
+                                            div_mode="reciprocal", occupancy=plain.dilate_cells(occ_dev),
+                                            t_stop=w.t_stop)
+    cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance, "trilinear", w.near, w.t_stop)
+    gd = w.grid.to(DEV)
+    bits = occupancy_dilate(pack_occupancy(occ_dev), w.dims)
+    gg = torch.full(w.grid.shape, 7.0, device=DEV)  # overwritten by the reduce
+    out = torch.empty(R, 4, device=DEV)
+    train_step(gd, gg, cfg=cfg, origins=od, dirs=dd, targets=tgt, out=out, occupancy=bits,
+               blocks=corner_blocks(gd), overwrite=True)
+    err = (out - want).abs().max().item()
+    ok, info = grad_close(gg, g.grad)
+    say("  rgba max err", err, "grad", info)
+    assert err <= 1e-5
+    assert ok, info
+
+
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 def test_sparse256_half_storage_error_report(dtype):
     say("sparse256 precision report", dtype)
