@@ -254,7 +254,7 @@ def cpu_baseline(S, args):
 
     def run():
         grid.grad = None
-        o, d, tgt = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam, u, v)
+        o, d, tgt = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam, u, v, norm_mode="gpu")
         plain.render_chunked_backward(grid, o, d, tgt, chunk_rays=n, num_samples=w.num_samples,
                                       delta_step=w.delta_step, gmin=gmin,
                                       points_distance=w.points_distance, interp=w.interp,

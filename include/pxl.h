@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PXL_ABI_VERSION 2
+#define PXL_ABI_VERSION 3
 
 /* return codes */
 #define PXL_OK 0
@@ -86,11 +86,18 @@ typedef struct pxl_march {
   float grid_min[3];  /* grid_indices.min(0) (src/ray_sampling.py:13) */
 } pxl_march;
 
+/* pxl_rays.flags */
+#define PXL_RAYS_CPU_NORM 1 /* camera-form directions normalised in the summation order of torch's
+                               CPU norm, sqrt(fma(z,z,fma(y,y,x*x))); default: the order of its
+                               GPU norm, sqrt((x*x + z*z) + y*y) -- the reference's default device
+                               is 'cuda' (src/ray_sampling.py:195,218,262) */
+
 /* A batch of rays, in one of two forms.
  * Explicit: origins/dirs (n_rays,3) f32.
  * Camera (origins == NULL): ray r comes from camera cam_ids[r] at uniforms uv[r] = (u,v) in
  * [0,1], exactly as src/ray_sampling.py:212-264 derives a direction and a target pixel from
- * a c2w matrix and camera_angle_x; its origin is c2w[cam][:3,3] (src/ray_sampling.py:159). */
+ * a c2w matrix and camera_angle_x; its origin is c2w[cam][:3,3] (src/ray_sampling.py:159).
+ * Every op is rounded as torch rounds it; the two 3-norms follow `flags`. */
 typedef struct pxl_rays {
   int64_t n_rays;
   const float* origins;   /* (n,3) or NULL for camera form */
@@ -100,7 +107,7 @@ typedef struct pxl_rays {
   const float* c2w;       /* (n_cams,4,4) row-major camera-to-world matrices */
   int64_t n_cams;
   float camera_angle_x;
-  int32_t reserved;
+  int32_t flags;          /* PXL_RAYS_* */
   const float* imgs;      /* (n_cams,H,W,4) target images or NULL */
   int64_t img_h, img_w;
 } pxl_rays;
@@ -142,8 +149,12 @@ typedef struct pxl_loss {
 } pxl_loss;
 
 /* One training step's render + loss gradient + backward in one pass: out = render(rays);
- * accumulates d loss / d grid into grad_grid.  out_rgba may be NULL.  n_samples <= 512.
+ * accumulates d loss / d grid into grad_grid.  out_rgba may be NULL.  n_samples <=
+ * PXL_TRAIN_MAX_SAMPLES: up to 512 samples every kept sample's value stays in registers
+ * between the forward and the reverse sweep; above that each ray keeps its per-group state
+ * (64 kept samples a group) in LDS and re-gathers each group on the reverse sweep.
  * Replaces scripts/train.py:130-177 + :180-181 minus the optimiser and the TV term. */
+#define PXL_TRAIN_MAX_SAMPLES 16384
 int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays* rays,
                    const pxl_loss* loss, float* out_rgba, float* grad_grid, void* stream);
 

@@ -60,21 +60,47 @@ def reciprocal_f32(points_distance):
 
 # --------------------------------------------------------------------------- rays
 
-def camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v):
+def norm3(x, norm_mode=None):
+    """x.norm(dim=-1) of (..., 3) float32 vectors (src/ray_sampling.py:218,262).
+
+    norm_mode None: torch's own norm on x's device (what the reference computes there).
+    "cpu": ATen's CPU reduction order, sqrt(fma(z, z, fma(y, y, x*x))) -- on a CPU tensor this
+    is torch's norm itself.  "gpu": ATen's GPU reduction order, sqrt((x*x + z*z) + y*y), each
+    op rounded to float32 (its reduce kernel gives a length-3 row two threads: elements 0 and
+    2 on one, 1 on the other) and a correctly rounded square root.  The root is taken in
+    float64 and rounded once to float32, which is the correctly rounded float32 root (53 >=
+    2*24 + 2 bits); torch's elementwise float32 sqrt on the CPU is not correctly rounded (it
+    differs from it on a few percent of inputs), while both of torch's GPU roots are.
+    Pinned by tests/test_oracle.py (CPU order vs torch on the CPU) and
+    tests/test_gpu_render.py::test_torch_gpu_norm_order (GPU order vs torch on the GPU,
+    bitwise over millions of vectors).
+    """
+    if norm_mode is None:
+        return x.norm(dim=-1)
+    if norm_mode == "cpu":
+        return x.cpu().norm(dim=-1).to(x.device)
+    if norm_mode == "gpu":
+        a, b, c = x[..., 0], x[..., 1], x[..., 2]
+        return torch.sqrt(((a * a + c * c) + b * b).double()).float()
+    raise ValueError(norm_mode)
+
+
+def camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v, norm_mode=None):
     """Per-ray form of src/ray_sampling.py:212-264 (generate_rays_batched).
 
     imgs (B,H,W,4), c2w (B,4,4), cam_ids (R,) int64, u/v (R,) in [0,1].  Returns
     origins (R,3) = c2w[cam,:3,3] (src/ray_sampling.py:159), unit directions (R,3) and
     the target pixels (R,4) imgs[cam, min(round(W*v),W-1), min(round(H*u),H-1)].
     Elementwise ops are applied per ray in the reference's order, so the values equal
-    the reference's (B,R)-shaped computation for the same (cam, u, v).
+    the reference's (B,R)-shaped computation for the same (cam, u, v).  norm_mode: the
+    summation order of the two 3-norms (see norm3; None = torch's on this device).
     Pinned by: reference golden vectors (generate_rays_batched), closed forms
     (unit norm, centre ray = -Z axis, pixel round/clamp).
     """
     cam_x = c2w[:, :3, 0]
     cam_y = c2w[:, :3, 1]
     cam_z = -c2w[:, :3, 2]
-    aspect = cam_x.norm(dim=1) / cam_y.norm(dim=1)
+    aspect = norm3(cam_x, norm_mode) / norm3(cam_y, norm_mode)
     H, W = imgs.shape[1], imgs.shape[2]
     col = (H * u).round().clamp(max=H - 1).to(torch.long)
     row = (W * v).round().clamp(max=W - 1).to(torch.long)
@@ -82,7 +108,7 @@ def camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v):
     us = camera_angle_x * (u - 0.5)
     vs = -((camera_angle_x * (1 / aspect))[cam_ids] * (v - 0.5))
     d = us.unsqueeze(-1) * cam_x[cam_ids] + vs.unsqueeze(-1) * cam_y[cam_ids] + cam_z[cam_ids]
-    d = d / d.norm(dim=1).unsqueeze(-1)
+    d = d / norm3(d, norm_mode).unsqueeze(-1)
     origins = c2w[cam_ids, :3, 3]
     return origins, d, targets
 
@@ -94,7 +120,7 @@ def even_spread_uniforms(number_of_rays, num_cameras, device="cpu"):
     return torch.cartesian_prod(ls, ls).repeat([num_cameras, 1, 1])
 
 
-def generate_rays_batched(imgs, number_of_rays, c2w, camera_angle_x, uniforms):
+def generate_rays_batched(imgs, number_of_rays, c2w, camera_angle_x, uniforms, norm_mode=None):
     """src/ray_sampling.py:195-264 with the uniforms (B,R,2) passed in instead of drawn.
 
     Returns (dirs (B*R,3), pixels_to_rays (B*R,4)) in camera-major order.
@@ -103,8 +129,14 @@ def generate_rays_batched(imgs, number_of_rays, c2w, camera_angle_x, uniforms):
     cam_ids = torch.repeat_interleave(torch.arange(B, device=imgs.device), R, 0)
     u = uniforms[:, :, 0].reshape(-1)
     v = uniforms[:, :, 1].reshape(-1)
-    _, d, tgt = camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v)
+    _, d, tgt = camera_rays(imgs, c2w, camera_angle_x, cam_ids, u, v, norm_mode)
     return d, tgt
+
+
+def device_norm_mode(div_mode):
+    """The norm order that goes with a division mode: both are the semantics of one device
+    (ATen CPU: exact division, CPU norm; ATen GPU: reciprocal multiply, GPU norm)."""
+    return {"divide": "cpu", "reciprocal": "gpu"}[div_mode]
 
 
 # --------------------------------------------------------------------------- samples
@@ -340,11 +372,16 @@ def tv_loss(input_tensor):
 
 
 def reference_fit(imgs, c2w, camera_angle_x, *, gridsize, points_distance, num_samples, delta_step, lr, tv,
-                  beta, uniforms, progressive_growing=True, div_mode="divide", start_step=0, init_grid=None):
+                  beta, uniforms, progressive_growing=True, div_mode="divide", start_step=0, init_grid=None,
+                  norm_mode="match"):
     """scripts/train.py:79-184 op for op on given per-step uniforms (list of (B,R,2)).
 
     Returns (grid, grid_grad_abs_sum, per-step loss dicts).  Rays: camera-major, R per camera.
+    div_mode / norm_mode: the division and 3-norm semantics ("match": the norm of the device
+    whose division div_mode names, see device_norm_mode; None: torch's on imgs' device).
     """
+    if norm_mode == "match":
+        norm_mode = device_norm_mode(div_mode)
     dev = imgs.device
     grid_indices, grid_full, _, grid_grid_original = generate_grid(*gridsize, points_distance=points_distance,
                                                                    device=dev)
@@ -368,7 +405,7 @@ def reference_fit(imgs, c2w, camera_angle_x, *, gridsize, points_distance, num_s
             gi = grid_grid_original.reshape(-1, 3)
             pd = points_distance
         B, R = uv.shape[0], uv.shape[1]
-        d, pix = generate_rays_batched(imgs, R, c2w, camera_angle_x, uv)
+        d, pix = generate_rays_batched(imgs, R, c2w, camera_angle_x, uv, norm_mode)
         o = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
         p = sample_positions(o, d, num_samples, delta_step)
         n = normalize_samples(p, grid_min(gi), pd, div_mode)

@@ -20,9 +20,11 @@ PXL_E_HIP = -4
 PXL_E_NO_DEVICE = -5
 
 DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
-ABI_VERSION = 2  # PXL_ABI_VERSION of the include/pxl.h these signatures mirror
+ABI_VERSION = 3  # PXL_ABI_VERSION of the include/pxl.h these signatures mirror
 INTERP_NEAREST, INTERP_TRILINEAR = 0, 1
 FLAG_EXACT_DIV = 1
+RAYS_CPU_NORM = 1
+TRAIN_MAX_SAMPLES = 16384
 
 c_void_p = ctypes.c_void_p
 
@@ -44,7 +46,7 @@ class Rays(ctypes.Structure):
     _fields_ = [("n_rays", ctypes.c_int64), ("origins", c_void_p), ("dirs", c_void_p),
                 ("cam_ids", c_void_p), ("uv", c_void_p), ("c2w", c_void_p),
                 ("n_cams", ctypes.c_int64), ("camera_angle_x", ctypes.c_float),
-                ("reserved", ctypes.c_int32), ("imgs", c_void_p),
+                ("flags", ctypes.c_int32), ("imgs", c_void_p),
                 ("img_h", ctypes.c_int64), ("img_w", ctypes.c_int64)]
 
 

@@ -227,6 +227,7 @@ struct KParams {
   float delta, near, t_stop, pd, inv_pd;
   float gx, gy, gz;
   int exact_div;
+  int cpu_norm;  // ray directions normalised in ATen's CPU summation order (else its GPU order)
   int aux_stride;
   int corner_blocks;  // trilinear gradients go to per-cell corner-block entries (16 floats)
   uint32_t occ_ty, occ_tz;  // occupancy tiles along y and z (ceil(Y/8), ceil(Z/8))
@@ -249,11 +250,23 @@ struct Ray {
   float u, v;
 };
 
-// three-element norm in the order torch's CPU reduction uses: sqrt(fma(z,z,fma(y,y,x*x))).
+// Three-element norms in the summation orders of torch's two reductions (x.norm(dim=-1) on a
+// length-3 axis, src/ray_sampling.py:218,262):
+//   ATen CPU: sqrt(fma(z,z,fma(y,y,x*x)))
+//   ATen GPU: sqrt((x*x + z*z) + y*y) -- the reduce kernel gives each output two threads,
+//             thread 0 summing elements 0 and 2 and thread 1 element 1, then combines them.
 // __builtin_sqrtf lowers to the correctly rounded sequence (v_sqrt_f32 plus the +-1 ulp
 // fma checks); __fsqrt_rn lowers to the bare v_sqrt_f32, which is not correctly rounded.
-__device__ __forceinline__ float norm3(float x, float y, float z) {
+__device__ __forceinline__ float norm3_cpu(float x, float y, float z) {
   return __builtin_sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
+}
+
+__device__ __forceinline__ float norm3_gpu(float x, float y, float z) {
+  return __builtin_sqrtf((x * x + z * z) + y * y);
+}
+
+__device__ __forceinline__ float norm3(bool cpu_order, float x, float y, float z) {
+  return cpu_order ? norm3_cpu(x, y, z) : norm3_gpu(x, y, z);
 }
 
 // src/ray_sampling.py:212-264 for one ray of the camera form (every op rounded as torch does).
@@ -265,13 +278,14 @@ __device__ __forceinline__ void camera_ray(const KParams& P, int64_t r, Ray& R) 
   const float xx = M[0], xy = M[4], xz = M[8];
   const float yx = M[1], yy = M[5], yz = M[9];
   const float zx = -M[2], zy = -M[6], zz = -M[10];
-  const float aspect = __fdiv_rn(norm3(xx, xy, xz), norm3(yx, yy, yz));
+  const bool cpu = P.cpu_norm != 0;
+  const float aspect = __fdiv_rn(norm3(cpu, xx, xy, xz), norm3(cpu, yx, yy, yz));
   const float us = P.theta * (u - 0.5f);
   const float vs = -((P.theta * __fdiv_rn(1.f, aspect)) * (v - 0.5f));
   const float dx = ((us * xx) + (vs * yx)) + zx;
   const float dy = ((us * xy) + (vs * yy)) + zy;
   const float dz = ((us * xz) + (vs * yz)) + zz;
-  const float nd = norm3(dx, dy, dz);
+  const float nd = norm3(cpu, dx, dy, dz);
   R.dx = __fdiv_rn(dx, nd);
   R.dy = __fdiv_rn(dy, nd);
   R.dz = __fdiv_rn(dz, nd);
@@ -588,11 +602,14 @@ __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int 
 // (compacted by rank), then the whole wave flushes them four lanes per cell, so one atomic
 // instruction covers 16 cells and each cell's four channels go out as one 16-byte group.
 // With RUNS, trilinear stages where at least a quarter of the samples share the previous
-// sample's floor cell (coarse grids: the pooled grids of progressive growing) are flushed by
-// runs instead: a sample's eight item indices depend only on its floor cell (item 0
-// identifies it), so each half-wave walks its 16 staged samples with one lane per (corner,
-// channel), sums a run of same-cell samples in sample order and adds the sum once -- two
-// memory requests per run instead of per sample.
+// sample's items (coarse grids: the pooled grids of progressive growing) are flushed by runs
+// instead: each half-wave walks its 16 staged samples with one lane per (corner, channel),
+// sums a run of samples with the same eight item indices in sample order and adds the sum
+// once -- two memory requests per run instead of per sample.  A sample's items are fixed by
+// its floor cell and by which coordinates are exact integers (there the ceil index equals the
+// floor index), so a run is keyed on item 0 (all-ceil corner) and item NI - 1 (all-floor
+// corner) together: the floor corner alone misses the integer case, the ceil corner alone
+// confuses a sample at integer z = c with its neighbour of floor c - 1.
 template <int INTERP, bool RUNS = false>
 __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, uint32_t clip, const float4& gv,
@@ -611,14 +628,18 @@ __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, in
     const int ns = min(nk - h, NS), n = ns * NI;
     bool by_runs = false;
     if constexpr (RUNS && INTERP == PXL_INTERP_TRILINEAR) {
-      const bool dup = lane >= 1 && lane < ns && s_cell[lane * NI] == s_cell[(lane - 1) * NI];
+      // same items as the previous sample: items 0 and NI - 1 both equal
+      auto same_items = [&](int a, int b) {
+        return s_cell[a * NI] == s_cell[b * NI] && s_cell[a * NI + NI - 1] == s_cell[b * NI + NI - 1];
+      };
+      const bool dup = lane >= 1 && lane < ns && same_items(lane, lane - 1);
       by_runs = 4 * __popcll(__ballot(dup)) >= ns;
       if (by_runs) {
         const int j = (lane >> 2) & 7, r0 = (lane >> 5) * (NS / 2), r1 = min(ns, r0 + NS / 2);
         float acc = 0.f;
         for (int r = r0; r < r1; ++r) {
           acc += sv[4 * (r * NI + j) + ch];
-          if (r + 1 == r1 || s_cell[(r + 1) * NI] != s_cell[r * NI]) {  // end of a run
+          if (r + 1 == r1 || !same_items(r + 1, r)) {  // end of a run
             if (acc != 0.f) atomic_add_f32(grad + (size_t)s_cell[r * NI + j] * 4u + ch, acc);
             acc = 0.f;
           }

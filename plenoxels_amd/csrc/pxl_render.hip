@@ -62,23 +62,33 @@ __device__ __forceinline__ bool stop_check(const KParams& P, bool valid, float T
   return true;
 }
 
-// Forward: out_rgba (n,4) and, when aux != NULL, the per-group state the backward needs.
-// Kept samples stream through a 128-entry LDS buffer; any S is handled.
+// Beta regulariser of scripts/train.py:170-174: f(a) = log(a + eps) - log(1 - a + eps) per
+// sample, eps = 1e-4, and its derivative.
+constexpr float kBetaEps = 1e-4f;
+__device__ __forceinline__ float beta_term(float a) {
+  return logf(a + kBetaEps) - logf(1.f - a + kBetaEps);
+}
+__device__ __forceinline__ float beta_dloss(float a) {
+  return 1.f / (a + kBetaEps) + 1.f / (1.f - a + kBetaEps);
+}
+
+// Forward sweep of one ray (one wave): its kept samples stream through `buf` (2 * kWave ints
+// of LDS) in groups of 64; returns the composite (every lane holds it).  When `grp` is set,
+// group g's start state goes to grp[2g] (transmittance before the group) and grp[2g + 1] (its
+// first sample index, as int bits), `ng` receives the group count.  n_proc: the bound of the
+// samples that may contribute (S unless early termination stopped the ray).  With `beta`,
+// bsum (every lane) = sum over processed samples of f(a) - f(0).  Any S is handled.
 template <int DT, int INTERP>
-__global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restrict__ out,
-                                                        float* __restrict__ aux) {
-  __shared__ int s_buf[kWavesPerBlock][2 * kWave];
-  const int lane = threadIdx.x & 63;
-  const int64_t ray = wave_ray(P);
-  if (ray >= P.n_rays) return;
-  int* buf = s_buf[threadIdx.x >> 6];
-  Ray R;
-  setup_ray(P, ray, R);
+__device__ __forceinline__ float4 forward_sweep(const KParams& P, const Ray& R, int lane, int* buf,
+                                                float* grp, int& ng, int& n_proc, bool beta, float& bsum) {
   const int S = P.n_samples, nch = (S + 63) >> 6;
-  float* ax = aux ? aux + ray * P.aux_stride : nullptr;
+  const float f0 = beta_term(0.f);
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
-  int n_proc = S, ng = 0, nbuf = 0;
+  float bs = 0.f;
+  int nbuf = 0;
   bool stop = false;
+  n_proc = S;
+  ng = 0;
   for (int c = 0; c < nch && !stop; ++c) {
     nbuf = append_kept<INTERP>(P, R, (c << 6) + lane + 1, S, buf, nbuf);
     const bool last = c == nch - 1;
@@ -87,9 +97,9 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
       const int m = min(nbuf, kWave);
       const bool valid = lane < m;
       const int k = valid ? buf[lane] : 0;
-      if (ax && lane == 0) {
-        ax[2 + 2 * ng] = T;
-        ax[3 + 2 * ng] = __int_as_float(k);
+      if (grp && lane == 0) {
+        grp[2 * ng] = T;
+        grp[2 * ng + 1] = __int_as_float(k);
       }
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       uint32_t clip;
@@ -99,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
       bool proc = valid;
       stop = stop_check(P, valid, Ti, k, lane, proc, n_proc);
       accumulate(proc, Ti, v, cr, cg, cb, ca);
+      if (beta && proc) bs += beta_term(v.w) - f0;
       T = T * readlane(incl, 63);
       ++ng;
       const int rest = nbuf - m;  // move the unprocessed tail to the front
@@ -109,12 +120,27 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
       wave_lds_sync();
     }
   }
-  cr = wave_sum(cr);
-  cg = wave_sum(cg);
-  cb = wave_sum(cb);
-  ca = wave_sum(ca);
+  if (beta) bsum = wave_sum(bs);
+  return make_float4(wave_sum(cr), wave_sum(cg), wave_sum(cb), wave_sum(ca));
+}
+
+// Forward: out_rgba (n,4) and, when aux != NULL, the per-group state the backward needs.
+template <int DT, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restrict__ out,
+                                                        float* __restrict__ aux) {
+  __shared__ int s_buf[kWavesPerBlock][2 * kWave];
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  Ray R;
+  setup_ray(P, ray, R);
+  float* ax = aux ? aux + ray * P.aux_stride : nullptr;
+  int ng, n_proc;
+  float bsum;
+  const float4 c = forward_sweep<DT, INTERP>(P, R, lane, s_buf[threadIdx.x >> 6], ax ? ax + 2 : nullptr, ng,
+                                             n_proc, false, bsum);
   if (lane == 0) {
-    reinterpret_cast<float4*>(out)[ray] = make_float4(cr, cg, cb, ca);
+    reinterpret_cast<float4*>(out)[ray] = c;
     if (ax) {
       ax[0] = __int_as_float(n_proc);
       ax[1] = __int_as_float(ng);
@@ -122,15 +148,6 @@ __global__ __launch_bounds__(kBlock) void k_render_fwd(KParams P, float* __restr
   }
 }
 
-// Beta regulariser of scripts/train.py:170-174: f(a) = log(a + eps) - log(1 - a + eps) per
-// sample, eps = 1e-4, and its derivative.
-constexpr float kBetaEps = 1e-4f;
-__device__ __forceinline__ float beta_term(float a) {
-  return logf(a + kBetaEps) - logf(1.f - a + kBetaEps);
-}
-__device__ __forceinline__ float beta_dloss(float a) {
-  return 1.f / (a + kBetaEps) + 1.f / (1.f - a + kBetaEps);
-}
 
 // One group of the reverse sweep: given the group's clipped values, transmittance and Qn
 // (the suffix value just behind the group), emit dL/dvalue per sample, scatter it, and
@@ -187,28 +204,19 @@ struct WaveLds {
   }
 };
 
-template <int DT, int INTERP>
-__global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* __restrict__ grad_out,
-                                                        const float* __restrict__ aux,
-                                                        float* __restrict__ grad) {
-  using L = WaveLds<INTERP, kWave>;
-  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
-  const int lane = threadIdx.x & 63;
-  const int64_t ray = wave_ray(P);
-  if (ray >= P.n_rays) return;
-  const float4 g = reinterpret_cast<const float4*>(grad_out)[ray];
-  if (g.x == 0.f && g.y == 0.f && g.z == 0.f && g.w == 0.f) return;
-  unsigned char* mem = s_mem[threadIdx.x >> 6];
-  int* list = L::list(mem);
-  Ray R;
-  setup_ray(P, ray, R);
-  const float* ax = aux + ray * P.aux_stride;
-  const int n_proc = __float_as_int(ax[0]);
-  const int ng = __float_as_int(ax[1]);
+// Reverse sweep of one ray (one wave) from its per-group start state grp (as forward_sweep
+// writes it; ng groups, samples k <= n_proc): group by group from the back, its kept samples
+// are found again (`list`, kWave ints of LDS, may alias the staging area), re-gathered, and
+// their gradients -- the loss gradient g through the compositing, plus the beta term's
+// per-sample derivative when beta_scale != 0 -- scattered into grad.
+template <int DT, int INTERP, bool RUNS>
+__device__ __forceinline__ void reverse_sweep(const KParams& P, const Ray& R, int lane, int* list,
+                                              const float* grp, int ng, int n_proc, const float4& g,
+                                              float beta_scale, float* grad, uint32_t* s_cell, float4* s_val) {
   float4 Qn = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int gi = ng - 1; gi >= 0; --gi) {
-    const float T0 = ax[2 + 2 * gi];
-    const int k0 = __float_as_int(ax[3 + 2 * gi]);
+    const float T0 = grp[2 * gi];
+    const int k0 = __float_as_int(grp[2 * gi + 1]);
     int n = 0;  // the group: the first 64 kept samples from k0 on (k <= n_proc)
     for (int kb = k0; n < kWave && kb <= n_proc; kb += kWave) {
       const int k = kb + lane;
@@ -228,8 +236,27 @@ __global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* _
     if (valid) eval_kept<DT, INTERP>(P, R, k, v, clip);
     const float incl = wave_prefix_mul(1.f - v.w);
     const float Ti = T0 * wave_shift_up(incl, 1.f);
-    reverse_chunk<INTERP>(P, R, lane, k, valid, v, clip, Ti, g, 0.f, Qn, grad, L::cells(mem), L::vals(mem));
+    reverse_chunk<INTERP, RUNS>(P, R, lane, k, valid, v, clip, Ti, g, beta_scale, Qn, grad, s_cell, s_val);
   }
+}
+
+template <int DT, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_render_bwd(KParams P, const float* __restrict__ grad_out,
+                                                        const float* __restrict__ aux,
+                                                        float* __restrict__ grad) {
+  using L = WaveLds<INTERP, kWave>;
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
+  const int lane = threadIdx.x & 63;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  const float4 g = reinterpret_cast<const float4*>(grad_out)[ray];
+  if (g.x == 0.f && g.y == 0.f && g.z == 0.f && g.w == 0.f) return;
+  unsigned char* mem = s_mem[threadIdx.x >> 6];
+  Ray R;
+  setup_ray(P, ray, R);
+  const float* ax = aux + ray * P.aux_stride;
+  reverse_sweep<DT, INTERP, false>(P, R, lane, L::list(mem), ax + 2, __float_as_int(ax[1]), __float_as_int(ax[0]), g,
+                                   0.f, grad, L::cells(mem), L::vals(mem));
 }
 
 // Training step for S <= 64*NG: every kept sample's value stays in registers between the
@@ -500,6 +527,46 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG <= 4 
       train_ray<DT, INTERP, NG>(P, LP, R1, ray0 + 1, tgt1, list1, n1, lane, L::cells(mem), L::vals(mem), out,
                                 grad);
   }
+}
+
+// Training step for more than 512 samples per ray (up to 64 * kLongGroups): one ray per
+// wavefront.  The forward sweep keeps each group's start state (transmittance, first sample)
+// in LDS, the loss gradient is formed, and the reverse sweep re-gathers group by group from
+// the back, as pxl_render_fwd + pxl_render_bwd would with the aux in LDS instead of memory.
+// Coarse grids (the pooled steps of progressive growing, where this sample count is the
+// reference's default of 600) keep tens of consecutive samples in one cell: the scatter
+// flushes such stages by runs.
+constexpr int kLongGroups = PXL_TRAIN_MAX_SAMPLES / kWave;
+
+template <int DT, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_train_long(KParams P, LossParams LP, float* __restrict__ out,
+                                                        float* __restrict__ grad) {
+  using L = WaveLds<INTERP, 2 * kWave>;  // the forward's sample buffer, then the scatter staging
+  __shared__ __align__(16) unsigned char s_mem[kWavesPerBlock][L::kBytes];
+  __shared__ float s_grp[kWavesPerBlock][2 * kLongGroups];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t ray = wave_ray(P);
+  if (ray >= P.n_rays) return;
+  unsigned char* mem = s_mem[wave];
+  float* grp = s_grp[wave];
+  Ray R;
+  setup_ray(P, ray, R);
+  const float4 tgt = LP.targets ? reinterpret_cast<const float4*>(LP.targets)[ray] : camera_target(P, R);
+  int ng, n_proc;
+  float bsum = 0.f;
+  const float4 c = forward_sweep<DT, INTERP>(P, R, lane, L::list(mem), grp, ng, n_proc, LP.beta != nullptr, bsum);
+  const float d0 = c.x - tgt.x, d1 = c.y - tgt.y, d2 = c.z - tgt.z, d3 = c.w - tgt.w;
+  if (lane == 0) {
+    if (out) reinterpret_cast<float4*>(out)[ray] = c;
+    if (LP.sq_err) LP.sq_err[ray] = ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+    if (LP.beta) LP.beta[ray] = bsum;
+  }
+  wave_lds_sync();  // lane 0's group records are read by every lane
+  const float gs = LP.grad_scale;
+  const float4 gl = make_float4(gs * d0, gs * d1, gs * d2, gs * d3);
+  reverse_sweep<DT, INTERP, true>(P, R, lane, L::list(mem), grp, ng, n_proc, gl, LP.beta_scale, grad,
+                                  L::cells(mem), L::vals(mem));
 }
 
 __global__ void k_camera_rays(KParams P, float* __restrict__ o, float* __restrict__ d,
@@ -925,6 +992,7 @@ int check_march(const pxl_march* m) {
 int check_rays(const pxl_rays* r, bool need_imgs) {
   if (!r) return fail(PXL_E_INVALID, "rays descriptor is NULL");
   if (r->n_rays < 0) return fail(PXL_E_INVALID, "n_rays is negative");
+  if ((r->flags & ~PXL_RAYS_CPU_NORM) != 0) return fail(PXL_E_INVALID, "unknown ray flags 0x%x", r->flags);
   if (r->n_rays == 0) return PXL_OK;  // an empty batch reads nothing (torch gives it NULL buffers)
   if (r->origins) {
     if (!r->dirs) return fail(PXL_E_INVALID, "origins given without dirs");
@@ -979,6 +1047,7 @@ pxl::KParams make_params(const pxl_grid* g, const pxl_march* m, const pxl_rays* 
   P.img_h = r->img_h;
   P.img_w = r->img_w;
   P.theta = r->camera_angle_x;
+  P.cpu_norm = (r->flags & PXL_RAYS_CPU_NORM) ? 1 : 0;
   return P;
 }
 
@@ -1090,9 +1159,9 @@ static int train_step(const pxl_grid* grid, const pxl_march* march, const pxl_ra
                 (long long)kMaxBlockCells);
   if (!std::isfinite(loss->grad_scale) || !std::isfinite(loss->beta_scale))
     return fail(PXL_E_INVALID, "grad_scale and beta_scale must be finite");
-  if (march->n_samples > 512)
-    return fail(PXL_E_INVALID, "pxl_train_step handles n_samples <= 512 (got %d); use "
-                "pxl_render_fwd + pxl_render_bwd", march->n_samples);
+  if (march->n_samples > PXL_TRAIN_MAX_SAMPLES)
+    return fail(PXL_E_INVALID, "pxl_train_step handles n_samples <= %d (got %d); use "
+                "pxl_render_fwd + pxl_render_bwd", PXL_TRAIN_MAX_SAMPLES, march->n_samples);
   if (rays->n_rays == 0) return PXL_OK;
   pxl::KParams P = make_params(grid, march, rays);
   P.corner_blocks = blocks ? 1 : 0;
@@ -1100,6 +1169,15 @@ static int train_step(const pxl_grid* grid, const pxl_march* march, const pxl_ra
                      loss->ray_beta};
   hipStream_t s = (hipStream_t)stream;
   const int S = march->n_samples;
+  if (S > 512) {  // one ray per wavefront, group state in LDS (k_train_long)
+    PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
+      rc = for_ray_pieces(P, pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {
+        hipLaunchKernelGGL((pxl::k_train_long<kDT, kIN>), dim3(nb), dim3(pxl::kBlock), 0, s, Q, LP, out_rgba,
+                           grad_grid);
+      }, blocks ? "pxl_train_step_blocks" : "pxl_train_step");
+    });
+    return rc;
+  }
   PXL_DISPATCH_DT_INTERP(grid->dtype, march->interp, {
     // two rays per wavefront (k_train_pair), eight per block
     rc = for_ray_pieces(P, 2 * pxl::kWavesPerBlock, [&](const pxl::KParams& Q, unsigned nb) {

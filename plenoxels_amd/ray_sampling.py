@@ -7,6 +7,9 @@ HIP kernels (include/pxl.h).  Differences, all at the reference's failure points
   * image indices from uniforms outside [0,1] (or non-square images) are clamped into the
     image where the reference would raise IndexError;
   * images must have 4 channels (RGBA, what the reference's loss compares against).
+Device semantics: the reference's default device is 'cuda', and these drop-ins reproduce what
+its ops compute there -- directions normalised with torch's GPU 3-norm order, division by
+points_distance as ATen's GPU kernel does it (a multiply by the float32 reciprocal).
 Gradients: compute_alpha_weighted_pixels is differentiable w.r.t. its samples, as in the
 reference's training chain; the ray/sample geometry outputs carry no gradient.
 """
@@ -68,14 +71,15 @@ def generate_rays_batched(imgs, number_of_rays, transform_matricies, camera_angl
     return rays_from_uniforms(imgs, transform_matricies, camera_angle_x, uv)
 
 
-def rays_from_uniforms(imgs, transform_matricies, camera_angle_x, uv):
-    """generate_rays_batched on given uniforms uv (B,R,2): (dirs (B*R,3), pixels (B*R,4))."""
+def rays_from_uniforms(imgs, transform_matricies, camera_angle_x, uv, norm_mode="gpu"):
+    """generate_rays_batched on given uniforms uv (B,R,2): (dirs (B*R,3), pixels (B*R,4)).
+    norm_mode: "gpu" (torch's GPU 3-norm, the reference on 'cuda') or "cpu" (its CPU norm)."""
     B, R = uv.shape[0], uv.shape[1]
     dev = _dev(transform_matricies)
     cam = CameraRays(torch.repeat_interleave(torch.arange(B, dtype=torch.int32, device=dev), R),
                      uv.reshape(B * R, 2).float().contiguous(),
                      transform_matricies.float().contiguous(), float(camera_angle_x),
-                     imgs.float().contiguous())
+                     imgs.float().contiguous(), norm_mode=norm_mode)
     _, d, tgt = camera_rays(cam, want_targets=True)
     return d, tgt
 

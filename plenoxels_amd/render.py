@@ -108,6 +108,7 @@ def _rays_struct(origins=None, dirs=None, camera=None):
     r.c2w = cam.c2w.data_ptr()
     r.n_cams = cam.c2w.shape[0]
     r.camera_angle_x = float(np.float32(cam.camera_angle_x))
+    r.flags = _lib.RAYS_CPU_NORM if cam.norm_mode == "cpu" else 0
     if cam.imgs is not None:
         r.imgs = cam.imgs.data_ptr()
         r.img_h = cam.imgs.shape[1]
@@ -118,14 +119,21 @@ def _rays_struct(origins=None, dirs=None, camera=None):
 @dataclass
 class CameraRays:
     """Rays in camera form: camera index and uniforms (u,v) per ray, plus the camera stack
-    (src/ray_sampling.py:195-264 with the uniforms drawn by the caller)."""
+    (src/ray_sampling.py:195-264 with the uniforms drawn by the caller).
+
+    norm_mode: summation order of the direction and aspect 3-norms -- "gpu" (default), torch's
+    GPU norm, sqrt((x*x + z*z) + y*y), what the reference computes on its default device
+    'cuda'; "cpu", torch's CPU norm, what it computes with device='cpu'."""
     cam_ids: torch.Tensor          # (R,) int32
     uv: torch.Tensor               # (R,2) float32 in [0,1]
     c2w: torch.Tensor              # (B,4,4) float32
     camera_angle_x: float
     imgs: torch.Tensor = None      # (B,H,W,4) float32 targets
+    norm_mode: str = "gpu"
 
     def validate(self):
+        if self.norm_mode not in ("gpu", "cpu"):
+            raise ValueError(f"norm_mode must be 'gpu' or 'cpu', got {self.norm_mode!r}")
         if self.cam_ids.dtype != torch.int32 or self.cam_ids.dim() != 1:
             raise ValueError("cam_ids must be (R,) int32")
         if self.uv.dtype != torch.float32 or tuple(self.uv.shape) != (self.cam_ids.shape[0], 2):
@@ -145,49 +153,66 @@ class CameraRays:
 
 class _RenderFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, grid, origins, dirs, cfg, occupancy):
+    def forward(ctx, grid, origins, dirs, cfg, occupancy, render_copy):
         L = _lib.lib()
+        src = grid if render_copy is None else render_copy  # the cells the kernels gather
         R = origins.shape[0]
         out = torch.empty((R, 4), dtype=torch.float32, device=grid.device)
         need_aux = ctx.needs_input_grad[0]
         aux = (torch.empty((R, int(L.pxl_aux_floats(int(cfg.num_samples)))), dtype=torch.float32,
                            device=grid.device) if need_aux else None)
-        g = _grid_struct(grid, occupancy)
+        g = _grid_struct(src, occupancy)
         m = cfg.to_struct()
         r = _rays_struct(origins, dirs)
         _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
                                     _lib.ptr(out), _lib.ptr(aux), _lib.stream_handle(grid.device)))
         ctx.cfg = cfg
         if need_aux:
-            ctx.save_for_backward(grid, origins, dirs, aux, occupancy)
+            ctx.save_for_backward(src, origins, dirs, aux, occupancy)
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
-        grid, origins, dirs, aux, occupancy = ctx.saved_tensors
+        src, origins, dirs, aux, occupancy = ctx.saved_tensors
         L = _lib.lib()
         grad_out = grad_out.contiguous().float()
-        grad = torch.zeros(grid.shape, dtype=torch.float32, device=grid.device)
-        g = _grid_struct(grid, occupancy)
+        # the gradient is accumulated in FP32 and returned in FP32: the differentiable grid is
+        # always an FP32 tensor (render_rays rejects half-precision leaves)
+        grad = torch.zeros(src.shape, dtype=torch.float32, device=src.device)
+        g = _grid_struct(src, occupancy)
         m = ctx.cfg.to_struct()
         r = _rays_struct(origins, dirs)
         _lib.check(L.pxl_render_bwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r),
                                     _lib.ptr(grad_out), _lib.ptr(aux), _lib.ptr(grad),
-                                    _lib.stream_handle(grid.device)))
-        return grad.to(grid.dtype), None, None, None, None
+                                    _lib.stream_handle(src.device)))
+        return grad, None, None, None, None, None
 
 
 def render_rays(grid, origins, dirs, *, num_samples, delta_step, grid_min, points_distance,
-                interp="nearest", near=0.0, t_stop=0.0, occupancy=None, div_mode="reciprocal"):
+                interp="nearest", near=0.0, t_stop=0.0, occupancy=None, div_mode="reciprocal",
+                render_copy=None):
     """(R,4) RGBA of the reference chain for explicit rays, differentiable w.r.t. grid.
 
-    grid: raw (X,Y,Z,4) parameter (float32, bfloat16 or float16; clipped inside);
-    origins/dirs: (R,3) float32.  grid_min: grid_indices.min(0)[0] (see grid_min_of).
-    occupancy: packed int32 bit words (see pxl.h); trilinear reads the floor cell's bit.
+    grid: raw (X,Y,Z,4) parameter (clipped inside); origins/dirs: (R,3) float32.
+    grid_min: grid_indices.min(0)[0] (see grid_min_of).  occupancy: packed int32 bit words
+    (see pxl.h); trilinear reads the floor cell's bit.
+    Precision: gradients are FP32.  A grid that requires grad must be float32; to render from
+    reduced-precision storage, pass the FP32 master as `grid` and a bfloat16/float16 copy of it
+    as `render_copy` -- the kernels gather from the copy (half the bytes) and the FP32 gradient
+    goes to the master.  A bfloat16/float16 `grid` is accepted for forward-only use.
     """
+    if grid.dtype != torch.float32 and grid.requires_grad and torch.is_grad_enabled():
+        raise TypeError(f"render_rays differentiates float32 grids (gradients stay FP32); got a {grid.dtype} "
+                        "leaf -- keep a float32 master as `grid` and pass the half copy as render_copy=")
+    if render_copy is not None:
+        if grid.dtype != torch.float32:
+            raise TypeError("render_copy goes with a float32 master grid")
+        if tuple(render_copy.shape) != tuple(grid.shape) or render_copy.device != grid.device:
+            raise ValueError("render_copy must have the grid's shape and device")
+        render_copy = render_copy.detach()
     cfg = MarchConfig(int(num_samples), float(delta_step), tuple(grid_min), float(points_distance),
                       interp, float(near), float(t_stop), div_mode)
-    return _RenderFn.apply(grid, origins.contiguous(), dirs.contiguous(), cfg, occupancy)
+    return _RenderFn.apply(grid, origins.contiguous(), dirs.contiguous(), cfg, occupancy, render_copy)
 
 
 def camera_rays(camera, want_targets=True):

@@ -78,6 +78,7 @@ class Trainer:
         self.theta = float(camera_angle_x)
         self.dims = tuple(int(x) for x in gridsize)
         self.pd = float(points_distance)
+        self.number_of_rays = number_of_rays  # as given: the .pth records it (scripts/train.py:201)
         self.R = int(np.round(np.sqrt(number_of_rays)) ** 2) if even_spread else int(number_of_rays)
         self.S = int(num_samples)
         self.delta = float(delta_step)
@@ -219,7 +220,7 @@ class Trainer:
         return out
 
     def save(self, save_path, device_name):
-        save_grid(save_path, self.grid, self.grad_abs, device=device_name, number_of_rays=self.R,
+        save_grid(save_path, self.grid, self.grad_abs, device=device_name, number_of_rays=self.number_of_rays,
                   num_samples=self.S, delta_step=self.delta, even_spread=self.even_spread, camera_ray=False,
                   points_distance=self.pd, gridsize=list(self.dims))
 

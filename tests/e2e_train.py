@@ -54,7 +54,8 @@ def view_rays(c2w, theta, res):
     B = c2w.shape[0]
     cam = torch.repeat_interleave(torch.arange(B), res * res)
     uvs = uv.repeat(B, 1)
-    o, d, _ = plain.camera_rays(torch.zeros(B, 1, 1, 4), c2w, theta, cam, uvs[:, 0].contiguous(), uvs[:, 1].contiguous())
+    o, d, _ = plain.camera_rays(torch.zeros(B, 1, 1, 4), c2w, theta, cam, uvs[:, 0].contiguous(), uvs[:, 1].contiguous(),
+                                norm_mode="gpu")
     return o.contiguous(), d.contiguous()
 
 

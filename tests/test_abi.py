@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
         assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
     assert set(_lib.SIGNATURES) == set(names)
-    assert L.pxl_abi_version() == _lib.ABI_VERSION == 2
+    assert L.pxl_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -38,7 +38,7 @@ def test_struct_layout_matches_header(tmp_path):
         "pxl_march": ["n_samples", "interp", "flags", "reserved", "delta", "near", "t_stop", "pd",
                       "inv_pd", "grid_min"],
         "pxl_rays": ["n_rays", "origins", "dirs", "cam_ids", "uv", "c2w", "n_cams",
-                     "camera_angle_x", "reserved", "imgs", "img_h", "img_w"],
+                     "camera_angle_x", "flags", "imgs", "img_h", "img_w"],
         "pxl_loss": ["targets", "grad_scale", "beta_scale", "ray_sq_err", "ray_beta"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pxl.h"', "int main(void){"]
@@ -124,7 +124,7 @@ def test_validation_errors_before_any_hip_call():
     r2.n_rays = 4
     assert _fwd(g2, m2, r2, out=0) == _lib.PXL_E_INVALID
     g2, m2, r2 = _valid()
-    m2.n_samples = 600
+    m2.n_samples = _lib.TRAIN_MAX_SAMPLES + 1
     r2.n_rays = 4
     loss = _lib.Loss()
     loss.grad_scale = 1.0
@@ -134,7 +134,12 @@ def test_validation_errors_before_any_hip_call():
     loss.targets = 0x6000
     rc = L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), ctypes.byref(loss),
                           None, ctypes.c_void_p(0x5000), None)
-    assert rc == _lib.PXL_E_INVALID and "512" in _lib.last_error()  # fused step: S <= 512
+    assert rc == _lib.PXL_E_INVALID and str(_lib.TRAIN_MAX_SAMPLES) in _lib.last_error()
+    r3 = _lib.Rays()
+    r3.n_rays = 4
+    r3.origins, r3.dirs = 0x2000, 0x3000
+    r3.flags = 2  # unknown ray flag
+    assert _fwd(g2, m2, r3) == _lib.PXL_E_INVALID and "flags" in _lib.last_error()
     assert L.pxl_train_step(ctypes.byref(g2), ctypes.byref(m2), ctypes.byref(r2), None, None,
                             ctypes.c_void_p(0x5000), None) == _lib.PXL_E_INVALID
     d3 = _lib.dims_arg((4, 4, 0))

@@ -32,7 +32,8 @@ def _rays(w, n=None):
     cam = w.cam_ids if n is None else w.cam_ids[:n]
     uv = w.uv if n is None else w.uv[:n]
     imgs = torch.zeros(w.c2w.shape[0], 1, 1, 4)
-    o, d, _ = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam.long(), uv[:, 0].contiguous(), uv[:, 1].contiguous())
+    o, d, _ = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam.long(), uv[:, 0].contiguous(), uv[:, 1].contiguous(),
+                                norm_mode="gpu")
     return o.contiguous(), d.contiguous()
 
 

@@ -47,13 +47,18 @@ def test_sample_camera_rays_batched_bitwise(even):
     torch.manual_seed(123)
     smp, pix, cpos, dirs = rs.sample_camera_rays_batched(c2w.to(DEV), 0.69, imgs.to(DEV), R, S, delta,
                                                          even_spread=even, camera_ray=False, device="cuda")
-    d_w, pix_w = plain.generate_rays_batched(imgs, R, c2w, 0.69, uv)
+    d_w, pix_w = plain.generate_rays_batched(imgs, R, c2w, 0.69, uv, norm_mode="gpu")
     o_w = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
     smp_w = plain.sample_positions(o_w, d_w, S, delta)
     assert torch.equal(dirs.cpu(), d_w)
     assert torch.equal(pix.cpu(), pix_w)
     assert torch.equal(smp.cpu(), smp_w)
     assert torch.equal(cpos.cpu(), c2w[:, :3, 3])
+    # the reference's own ops on the GPU (its default device) with the same RNG draw
+    d_g, pix_g = plain.generate_rays_batched(imgs.to(DEV), R, c2w.to(DEV), 0.69, uv.to(DEV))
+    smp_g = plain.sample_positions(o_w.to(DEV), d_g, S, delta)
+    assert torch.equal(dirs.cpu(), d_g.cpu()) and torch.equal(pix.cpu(), pix_g.cpu())
+    assert torch.equal(smp.cpu(), smp_g.cpu())
     with pytest.raises(NotImplementedError):
         rs.sample_camera_rays_batched(c2w.to(DEV), 0.69, imgs.to(DEV), R, S, delta, False, True)
 
@@ -348,17 +353,19 @@ def _fit_inputs(B=3, H=24, R=64, steps=4, seed=20):
     return c2w, imgs, uniforms
 
 
+@pytest.mark.parametrize("S,delta", [(48, 0.05), (600, 0.005)])
 @pytest.mark.parametrize("start,dims,progressive", [(0, (24, 24, 24), False), (205, (48, 48, 48), True)])
-def test_trainer_matches_reference_loop(start, dims, progressive):
+def test_trainer_matches_reference_loop(start, dims, progressive, S, delta):
     """Trainer.step (fused) vs the reference loop restated op for op (oracle.plain.reference_fit),
-    same uniforms: full-grid steps, and progressive-growing steps with k = 13..11 pooling."""
+    same uniforms: full-grid steps, and progressive-growing steps with k = 13..11 pooling; 48
+    samples per ray, and the reference's default of 600 (scripts/train.py:218)."""
     from plenoxels_amd.train import Trainer
-    say("trainer vs reference loop", start, dims)
+    say("trainer vs reference loop", start, dims, S)
     c2w, imgs, uniforms = _fit_inputs()
     # a non-constant start grid: from all zeros the first TV step (k < 19) divides by
     # sqrt(0) and both loops turn NaN, as the reference does
     init = torch.rand(*dims, 4, generator=torch.Generator().manual_seed(30)) * 0.6 - 0.1
-    kw = dict(gridsize=dims, points_distance=0.06, num_samples=48, delta_step=0.05, lr=0.01, tv=1e-3,
+    kw = dict(gridsize=dims, points_distance=0.06, num_samples=S, delta_step=delta, lr=0.01, tv=1e-3,
               beta=5e-3, init_grid=init)
     grid_w, gabs_w, hist_w = plain.reference_fit(imgs, c2w, 0.69, uniforms=uniforms, div_mode="reciprocal",
                                                  progressive_growing=progressive, start_step=start, **kw)

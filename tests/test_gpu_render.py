@@ -62,11 +62,6 @@ def test_torch_gpu_scalar_division_is_reciprocal_multiply():
     for pd in (0.0125, 0.1, 3 / 255, 3 / 127, 0.3):
         got = (x.to(DEV) / pd).cpu()
         assert torch.equal(got, x * plain.reciprocal_f32(pd)), pd
-    # informational: does the GPU's 3-norm use the CPU's sequential-FMA order?
-    v = torch.randn(1 << 16, 3, generator=g)
-    cpu = v.norm(dim=1)
-    gpu = v.to(DEV).norm(dim=1).cpu()
-    say("torch GPU vs CPU 3-norm mismatches:", int((cpu != gpu).sum()))
 
 
 @pytest.mark.parametrize("interp", ["nearest", "trilinear"])
@@ -233,8 +228,13 @@ def test_transmittance_scan_fp64_on_gpu_near_saturation(interp):
     assert err <= 1e-5 and terr <= 1e-5
 
 
-def test_camera_rays_bitwise_vs_plain():
-    say("camera rays")
+@pytest.mark.parametrize("norm_mode", ["gpu", "cpu"])
+def test_camera_rays_bitwise_vs_plain(norm_mode):
+    """pxl_camera_rays vs the reference's ray generation restated (oracle.plain.camera_rays) on
+    the CPU with the same norm order, bitwise; in the default "gpu" mode also vs the reference's
+    own ops run on the GPU (torch's GPU norm), which is what the reference computes on its
+    default device."""
+    say("camera rays", norm_mode)
     B, H, W, R = 5, 37, 37, 4096
     c2w = random_cameras(B, 70, y_scale=1.3)
     g = torch.Generator().manual_seed(71)
@@ -244,14 +244,44 @@ def test_camera_rays_bitwise_vs_plain():
     uv[:8] = torch.tensor([[0.0, 0.0], [1.0, 1.0], [0.5, 0.5], [0.5 / H, 0.5 / W],
                            [1.5 / H, 2.5 / W], [1.0, 0.0], [0.0, 1.0], [0.25, 0.75]])
     theta = 0.6911112070083618
-    o_w, d_w, t_w = plain.camera_rays(imgs, c2w, theta, cam, uv[:, 0].contiguous(), uv[:, 1].contiguous())
-    cr = CameraRays(cam.to(torch.int32).to(DEV), uv.to(DEV), c2w.to(DEV), theta, imgs.to(DEV))
+    o_w, d_w, t_w = plain.camera_rays(imgs, c2w, theta, cam, uv[:, 0].contiguous(), uv[:, 1].contiguous(),
+                                      norm_mode=norm_mode)
+    cr = CameraRays(cam.to(torch.int32).to(DEV), uv.to(DEV), c2w.to(DEV), theta, imgs.to(DEV), norm_mode=norm_mode)
     o, d, t = camera_rays(cr)
     assert torch.equal(o.cpu(), o_w)
     assert torch.equal(t.cpu(), t_w)
     mism = int((d.cpu() != d_w).any(dim=1).sum())
-    say("  direction mismatches (bitwise):", mism)
+    say("  direction mismatches vs the restatement (bitwise):", mism)
     assert mism == 0
+    if norm_mode == "gpu":
+        _, d_g, t_g = plain.camera_rays(imgs.to(DEV), c2w.to(DEV), theta, cam.to(DEV), uv[:, 0].contiguous().to(DEV),
+                                        uv[:, 1].contiguous().to(DEV))
+        mism = int((d.cpu() != d_g.cpu()).any(dim=1).sum())
+        say("  direction mismatches vs torch's ops on the GPU (bitwise):", mism)
+        assert mism == 0 and torch.equal(t.cpu(), t_g.cpu())
+
+
+def test_torch_gpu_norm_order():
+    """Pins oracle.plain.norm3's "gpu" order: torch's GPU x.norm(dim=-1) over a length-3 axis is
+    sqrt((x*x + z*z) + y*y) with every op rounded to float32 -- for contiguous rows, for the
+    reference's (B,R,3) direction tensor and for strided camera-matrix columns
+    (src/ray_sampling.py:218,262); its CPU norm is the fma order, which differs on many rows."""
+    say("torch GPU 3-norm order")
+    g = torch.Generator().manual_seed(5)
+    v = torch.randn(1 << 22, 3, generator=g) * torch.rand(1 << 22, 1, generator=g) * 10
+    vd = v.to(DEV)
+    got = vd.norm(dim=1).cpu()
+    assert torch.equal(got, plain.norm3(v, "gpu"))
+    assert torch.equal(plain.norm3(vd, "gpu").cpu(), got)
+    diff_cpu = int((v.norm(dim=1) != got).sum())
+    say("  rows where the CPU order differs:", diff_cpu)
+    assert diff_cpu > 0  # the two orders are genuinely different
+    m = torch.randn(257, 4, 4, generator=g)
+    for col in range(3):  # strided (B,3) views, as camera_x_axis / camera_y_axis
+        s = m.to(DEV)[:, :3, col]
+        assert torch.equal(s.norm(dim=1).cpu(), plain.norm3(m[:, :3, col], "gpu"))
+    d3 = v[: 100 * 4096].reshape(100, 4096, 3)
+    assert torch.equal(d3.to(DEV).norm(dim=2).cpu(), plain.norm3(d3, "gpu"))
 
 
 def test_pack_occupancy_matches_oracle_bits():
@@ -275,7 +305,7 @@ def test_sparse256_subset_train_step_vs_plain(t_stop, fused_camera):
     cam, uv = w.cam_ids[:n].contiguous(), w.uv[:n].contiguous()
     imgs = torch.rand(w.c2w.shape[0], 800, 800, 4, generator=torch.Generator().manual_seed(3))
     o, d, tgt = plain.camera_rays(imgs, w.c2w, w.camera_angle_x, cam.long(), uv[:, 0].contiguous(),
-                                  uv[:, 1].contiguous())
+                                  uv[:, 1].contiguous(), norm_mode="gpu")
     occ_cells = plain.dilate_cells(w.occupancy)
     kw = dict(num_samples=w.num_samples, delta_step=w.delta_step, points_distance=w.points_distance,
               interp="trilinear", near=w.near)

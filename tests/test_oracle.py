@@ -428,3 +428,21 @@ def test_tv_loss_closed_form():
     # linear ramp: differences are the strides 4 (z), 20 (y), 80 (x)
     want = math.sqrt(4 * (3 * 4 * 4 * 16 + 3 * 3 * 5 * 400 + 2 * 4 * 5 * 6400))
     assert float(plain.tv_loss(g)) == pytest.approx(want)
+
+
+def test_norm3_orders_are_within_an_ulp_and_distinct():
+    """Both restated 3-norm orders are the float32 norm to within two ulps of the FP64 value (three
+    products and sums rounded, then a correctly rounded root), agree with torch's CPU norm in
+    the "cpu" order, and genuinely differ from each other on some vectors -- a swapped or dropped
+    term fails one of these."""
+    g = torch.Generator().manual_seed(12)
+    v = torch.randn(1 << 18, 3, generator=g) * torch.rand(1 << 18, 1, generator=g) * 5
+    exact = v.double().norm(dim=1)
+    for mode in ("cpu", "gpu"):
+        got = plain.norm3(v, mode).double()
+        assert ((got - exact).abs() <= 2.0 ** -22 * exact).all(), mode
+    assert torch.equal(plain.norm3(v, "cpu"), v.norm(dim=1))
+    assert int((plain.norm3(v, "cpu") != plain.norm3(v, "gpu")).sum()) > 100
+    # "gpu" is the x, z pair summed first: swapping y and z changes it, swapping x and z does not
+    assert torch.equal(plain.norm3(v[:, [2, 1, 0]], "gpu"), plain.norm3(v, "gpu"))
+    assert not torch.equal(plain.norm3(v[:, [0, 2, 1]], "gpu"), plain.norm3(v, "gpu"))
