@@ -168,6 +168,18 @@ int pxl_train_step(const pxl_grid* grid, const pxl_march* march, const pxl_rays*
 int pxl_scaled_sum(const float* v, int64_t n, double scale, float* out, void* work, void* stream);
 int64_t pxl_scaled_sum_work_bytes(void);
 
+/* pxl_scaled_sum without the caller's work buffer: the same kernel and the same bits, with its
+ * partials and ticket in a scratch array the library keeps in each GPU's memory
+ * (pxl_scaled_sum_noalloc_scratch_bytes() = 1 KiB per GPU, held until the process ends).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc -- and it
+ * allocates nothing itself.  Any n >= 0 fits the scratch (its size does not depend on n).  The
+ * first call on a device looks the scratch up and clears it with a kernel on `stream`; every
+ * call leaves it ready for the next.  All calls on one GPU share the one scratch: calls that
+ * overlap on one GPU (on different streams, or from different threads without ordering) race. */
+int pxl_scaled_sum_noalloc(const float* v, int64_t n, double scale, float* out, void* stream);
+int64_t pxl_scaled_sum_noalloc_scratch_bytes(void);
+
 /* pxl_train_step with the trilinear gradient accumulated into corner blocks instead of
  * grad_grid: blocks is (X*Y*Z, 16) f32 (pxl_blocks_floats), zero on entry: one 64-byte entry
  * per cell E = (x, y, z), whose slot s = 2bx + by (floats 4s + c) receives channel c of corner

@@ -74,6 +74,8 @@ SIGNATURES = {
     "pxl_scaled_sum": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_double, c_void_p, c_void_p,
                                       c_void_p]),
     "pxl_scaled_sum_work_bytes": (ctypes.c_int64, []),
+    "pxl_scaled_sum_noalloc": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_double, c_void_p, c_void_p]),
+    "pxl_scaled_sum_noalloc_scratch_bytes": (ctypes.c_int64, []),
     "pxl_blocks_reduce": (ctypes.c_int, [_GP, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_camera_rays": (ctypes.c_int, [_RP, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pxl_blocks_clear": (ctypes.c_int, [_GP, c_void_p, c_void_p]),

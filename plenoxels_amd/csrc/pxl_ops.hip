@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 
 #include "pxl.h"
@@ -755,8 +756,21 @@ __global__ __launch_bounds__(256) void k_scaled_sum(const float4* __restrict__ v
   }
 }
 
+constexpr int kNoAllocScratchBytes = 1024;
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, int n) {
+  const int i = (int)threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 }  // namespace
 }  // namespace pxl
+
+// Scratch of pxl_scaled_sum_noalloc: the partials and ticket of k_scaled_sum in the library's own
+// device memory, 1 KiB (each GPU holds one copy, at its own address, until the process ends).
+// Host code never uses this name as an address -- there it names a host-side stand-in -- only
+// the per-device pointer hipGetSymbolAddress returns for it.
+static __device__ __align__(256) unsigned char scratch[pxl::kNoAllocScratchBytes];
 
 // ============================================================================= host ABI
 
@@ -779,6 +793,46 @@ int dims3(const int64_t dims[3], Dims& d) {
   int rc = pxl_host::check_dims(dims[0], dims[1], dims[2]);
   if (rc) return rc;
   d = Dims{dims[0], dims[1], dims[2]};
+  return PXL_OK;
+}
+
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the device that runs `stream` (NULL: the current device),
+// looked up once per device with that device current; its first use on a device also clears it
+// (the ticket must start at zero) with a fill kernel on `stream`, which may be captured in a
+// graph (no memset node).
+int scratch_for(hipStream_t stream, void** out) {
+  int cur = 0, dev = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return pxl_host::hip_fail(e, "pxl_scaled_sum_noalloc: hipGetDevice");
+  dev = cur;
+  if (stream != nullptr) {
+    e = hipStreamGetDevice(stream, &dev);
+    if (e != hipSuccess) return pxl_host::hip_fail(e, "pxl_scaled_sum_noalloc: hipStreamGetDevice");
+  }
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(PXL_E_INVALID, "pxl_scaled_sum_noalloc: device %d beyond the %d it keeps scratch for", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    if (dev != cur && (e = hipSetDevice(dev)) != hipSuccess) return pxl_host::hip_fail(e, "pxl_scaled_sum_noalloc: hipSetDevice");
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p != nullptr) {
+      hipLaunchKernelGGL(pxl::k_zero_words, dim3(1), dim3(pxl::kNoAllocScratchBytes / 4), 0, stream,
+                         reinterpret_cast<uint32_t*>(p), pxl::kNoAllocScratchBytes / 4);
+      e = hipGetLastError();
+    }
+    if (dev != cur) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return pxl_host::hip_fail(e, "pxl_scaled_sum_noalloc: scratch set-up");
+    if (p == nullptr) return fail(PXL_E_HIP, "pxl_scaled_sum_noalloc: hipGetSymbolAddress gave NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
   return PXL_OK;
 }
 
@@ -1038,6 +1092,24 @@ int pxl_scaled_sum(const float* v, int64_t n, double scale, float* out, void* wo
   hipLaunchKernelGGL(pxl::k_scaled_sum, dim3(pxl::kSumBlocks), dim3(256), 0, (hipStream_t)stream,
                      reinterpret_cast<const float4*>(v), v, n, scale, out, partial, ticket);
   return launched("pxl_scaled_sum");
+}
+
+int64_t pxl_scaled_sum_noalloc_scratch_bytes(void) { return pxl::kNoAllocScratchBytes; }
+
+int pxl_scaled_sum_noalloc(const float* v, int64_t n, double scale, float* out, void* stream) {
+  if (n < 0) return fail(PXL_E_INVALID, "n must be >= 0");
+  if (!out || (n > 0 && !v)) return fail(PXL_E_INVALID, "v and out are required");
+  if (reinterpret_cast<uintptr_t>(v) % 16 != 0) return fail(PXL_E_INVALID, "v must be 16-byte aligned");
+  static_assert(pxl::kSumBlocks * sizeof(double) + sizeof(uint32_t) <= (size_t)pxl::kNoAllocScratchBytes,
+                "scratch layout");
+  void* work = nullptr;
+  int rc = scratch_for((hipStream_t)stream, &work);
+  if (rc) return rc;
+  double* partial = reinterpret_cast<double*>(work);
+  uint32_t* ticket = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(work) + pxl::kSumBlocks * sizeof(double));
+  hipLaunchKernelGGL(pxl::k_scaled_sum, dim3(pxl::kSumBlocks), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(v), v, n, scale, out, partial, ticket);
+  return launched("pxl_scaled_sum_noalloc");
 }
 
 int pxl_tv_sumsq(const float* grid, int64_t X, int64_t Y, int64_t Z, double* sum_sq, void* stream) {

@@ -257,6 +257,12 @@ def test_scaled_sum_validation():
     assert L.pxl_scaled_sum(ctypes.c_void_p(0x3004), 5, 1.0, out, work, None) == _lib.PXL_E_INVALID
     assert "aligned" in _lib.last_error()
     assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), 5, 1.0, out, ctypes.c_void_p(0x1008), None) == _lib.PXL_E_INVALID
+    # the allocation-free variant validates the same way, before it looks up its scratch
+    assert L.pxl_scaled_sum_noalloc(ctypes.c_void_p(0x3000), -1, 1.0, out, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum_noalloc(None, 5, 1.0, out, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum_noalloc(ctypes.c_void_p(0x3000), 5, 1.0, None, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum_noalloc(ctypes.c_void_p(0x3004), 5, 1.0, out, None) == _lib.PXL_E_INVALID
+    assert L.pxl_scaled_sum_noalloc_scratch_bytes() == 1024
 
 
 def test_no_device_is_reported_not_aborted():
@@ -268,6 +274,8 @@ def test_no_device_is_reported_not_aborted():
     rc = _fwd(g, m, r)
     assert rc in (_lib.PXL_E_NO_DEVICE, _lib.PXL_E_HIP)
     assert _lib.last_error()
+    rc = _lib.lib().pxl_scaled_sum_noalloc(ctypes.c_void_p(0x3000), 5, 1.0, ctypes.c_void_p(0x4000), None)
+    assert rc in (_lib.PXL_E_NO_DEVICE, _lib.PXL_E_HIP) and "noalloc" in _lib.last_error()
 
 
 def test_python_wrappers_reject_cpu_tensors():
