@@ -196,6 +196,16 @@ int pxl_train_step_blocks(const pxl_grid* grid, const pxl_march* march, const px
 /* Floats of the corner blocks of a grid (X*Y*Z*16), or -1 for invalid dims. */
 int64_t pxl_blocks_floats(const pxl_grid* grid);
 
+/* pxl_blocks_reduce modes */
+#define PXL_REDUCE_OVERWRITE 0            /* every cell of grad_grid written */
+#define PXL_REDUCE_ACCUMULATE 1           /* every cell added to */
+#define PXL_REDUCE_OVERWRITE_REACHABLE 2  /* only the cells some written entry reaches (with
+                                             grid->occupancy: those with a source entry of a set
+                                             bit); the others are not touched.  For a gradient
+                                             buffer that holds zeros outside that reach -- zeroed
+                                             once and used with one occupancy -- the result equals
+                                             PXL_REDUCE_OVERWRITE without its dense store pass. */
+
 /* grad_grid (X,Y,Z,4) f32 = (accumulate: +=) the sum, for every cell (x, y, z), of slot
  * 2bx + by of entry (x - 1 + bx, y - 1 + by, z) over bx, by in {0, 1} (corner (bx, by) of row
  * (x, y) names cell (x + 1 - bx, y + 1 - by), the ceil of the last index wrapping to 0 as
@@ -208,7 +218,7 @@ int64_t pxl_blocks_floats(const pxl_grid* grid);
  * pass); NULL: none.  Only those same entries are zeroed when occupancy is set, so keep the
  * occupancy of a block buffer fixed or zero it whole when it changes. */
 int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int32_t accumulate,
-                      float* clear, void* stream);
+                      float* clear, void* stream);  /* accumulate: PXL_REDUCE_* */
 
 /* Zero the entries of corner blocks that a training step with grid->occupancy can write (every
  * entry when occupancy is NULL) -- the clearing pxl_blocks_reduce does with clear == blocks,
@@ -247,6 +257,14 @@ int64_t pxl_occupancy_words(int64_t X, int64_t Y, int64_t Z);
 
 /* dst[0..n) = value (HBM-rate fill, e.g. the per-step gradient zero-fill). */
 int pxl_fill_f32(float* dst, int64_t n, float value, void* stream);
+
+/* The gradient zero-fill of a training step with grid->occupancy: grad (X,Y,Z,4) f32 = value
+ * in every 8x8x8 tile that holds a cell the step's scatter can write -- nearest: a tile with an
+ * occupied cell; trilinear (corners one cell above the floor cell, wrapped): a tile whose own
+ * bits or those of a lower neighbour tile (x-1, y-1, z-1, wrapped) are set.  Other cells are not
+ * touched: a gradient buffer zeroed once and used with one occupancy then holds exactly what a
+ * dense zero-fill would give it.  grid->data is not read; grid->occupancy is required. */
+int pxl_fill_reachable_f32(const pxl_grid* grid, int32_t interp, float* grad, float value, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Per-function kernels behind the reference's individual call signatures.  Unless stated
@@ -374,12 +392,6 @@ int pxl_adam_tv_step(const float* param_in, float* param_out, float* grad, float
 int pxl_count_cells_host(const pxl_grid* grid, const pxl_march* march, const float* origins,
                          const float* dirs, int64_t n_rays, const int32_t* n_proc,
                          int64_t* counts, int32_t n_threads);
-
-/* Test hook (no counterpart in the reference): runs the wave primitives the kernels are
- * built from on one wavefront.  in: 6*64 floats (x, then affine-map fields a0,a1,a2,a3,b);
- * out: 7*64 floats (inclusive product, exclusive product, wave sum, suffix-composed a0,
- * a3, b, and x shifted down one lane with -1 filling lane 63). */
-int pxl_debug_wave_ops(const float* in, float* out, void* stream);
 
 #ifdef __cplusplus
 }

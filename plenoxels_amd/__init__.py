@@ -6,7 +6,7 @@ kernels in libpxl.so (C ABI: include/pxl.h).  There is no CPU fallback: every en
 raises if the library is not built or no HIP device is present.
 """
 from .render import (CameraRays, CornerBlockPair, LossMean, MarchConfig, alpha_occupancy, blocks_clear, blocks_reduce, camera_rays, corner_blocks, fill_,  # noqa: F401
-                     grid_min_of, occupancy_dilate, pack_occupancy, render_rays, train_step)
+                     fill_reachable_, grid_min_of, occupancy_dilate, pack_occupancy, render_rays, train_step)
 
-__all__ = ["CameraRays", "CornerBlockPair", "LossMean", "MarchConfig", "alpha_occupancy", "blocks_clear", "blocks_reduce", "camera_rays", "corner_blocks", "fill_", "grid_min_of",
+__all__ = ["CameraRays", "CornerBlockPair", "LossMean", "MarchConfig", "alpha_occupancy", "blocks_clear", "blocks_reduce", "camera_rays", "corner_blocks", "fill_", "fill_reachable_", "grid_min_of",
            "occupancy_dilate", "pack_occupancy", "render_rays", "train_step"]

@@ -11,6 +11,7 @@ import torch  # noqa: F401  (loads the HIP runtime libpxl binds to)
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpxl.so")
+PROBE_LIB_PATH = os.path.join(LIB_DIR, "libpxl_probe.so")
 
 PXL_OK = 0
 PXL_E_INVALID = -1
@@ -23,6 +24,7 @@ DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
 ABI_VERSION = 3  # PXL_ABI_VERSION of the include/pxl.h these signatures mirror
 INTERP_NEAREST, INTERP_TRILINEAR = 0, 1
 FLAG_EXACT_DIV = 1
+REDUCE_OVERWRITE, REDUCE_ACCUMULATE, REDUCE_OVERWRITE_REACHABLE = 0, 1, 2
 RAYS_CPU_NORM = 1
 TRAIN_MAX_SAMPLES = 16384
 
@@ -86,6 +88,7 @@ SIGNATURES = {
     "pxl_occupancy_words": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "pxl_alpha_occupancy": (ctypes.c_int, [_GP, ctypes.c_int32, c_void_p, c_void_p]),
     "pxl_fill_f32": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p]),
+    "pxl_fill_reachable_f32": (ctypes.c_int, [_GP, ctypes.c_int32, c_void_p, ctypes.c_float, c_void_p]),
     "pxl_sample_points": (ctypes.c_int, [c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64,
                                          ctypes.c_int32, ctypes.c_float, ctypes.c_float, c_void_p,
                                          c_void_p]),
@@ -130,7 +133,13 @@ SIGNATURES = {
                                         ctypes.c_int64, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p]),
     "pxl_count_cells_host": (ctypes.c_int, [_GP, _MP, c_void_p, c_void_p, ctypes.c_int64,
                                             c_void_p, c_void_p, ctypes.c_int32]),
-    "pxl_debug_wave_ops": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+}
+
+# libpxl_probe (include/pxl_probe.h): measurement and test kernels, not loaded by the package
+PROBE_SIGNATURES = {
+    "pxl_probe_wave_ops": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "pxl_probe_memory_rate": (ctypes.c_int, [c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, c_void_p, c_void_p]),
 }
 
 
@@ -160,6 +169,24 @@ def lib():
             raise ImportError(f"libpxl ABI version {L.pxl_abi_version()}, these bindings expect {ABI_VERSION}")
         _lib = L
     return _lib
+
+
+_probe = None
+
+
+def probe_lib():
+    """Load libpxl_probe.so (measurement and test kernels, built beside libpxl.so)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_LIB_PATH):
+            raise ImportError(f"{PROBE_LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(PROBE_LIB_PATH)
+        for name, (res, args) in PROBE_SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _probe = L
+    return _probe
 
 
 def last_error():

@@ -701,25 +701,50 @@ __global__ void k_fill(float4* __restrict__ dst, int64_t n4, float value) {
     dst[i] = v;
 }
 
+// Fill of the 8x8x8 tiles a scatter can reach (pxl_fill_reachable_f32): one workgroup per tile.
+// Lanes 0..15 of wave 0 read the tile's own occupancy words (TRI: also those of the seven tiles
+// at -1 along any subset of x, y, z, wrapped -- a floor cell there has corners in this tile);
+// if any bit is set the 256 threads fill the tile's cells, two 16-byte stores each, a z row of
+// eight cells (128 bytes) per eight threads.  Cells beyond the dims (partial tiles) are skipped.
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_fill_reachable(const uint32_t* __restrict__ occ, uint32_t X, uint32_t Y,
+                                                        uint32_t Z, float4* __restrict__ grad, float value) {
+  __shared__ uint32_t s_any;
+  const uint32_t TX = (X + 7u) >> 3, TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
+  const uint32_t t = blockIdx.x;
+  const uint32_t tz = t % TZ, ty = (t / TZ) % TY, tx = t / (TZ * TY);
+  if (threadIdx.x < 64) {
+    uint32_t w = 0u;
+    const int lane = threadIdx.x;
+    constexpr int kWords = TRI ? 8 * 16 : 16;  // word i: tile offset j = i >> 4 in {0, -1}^3, word i & 15
+#pragma unroll
+    for (int i0 = 0; i0 < kWords; i0 += 64) {
+      const int i = i0 + lane;
+      if (i < kWords) {
+        const int j = i >> 4;
+        const uint32_t ax = (j & 4) ? (tx == 0 ? TX - 1u : tx - 1u) : tx;
+        const uint32_t ay = (j & 2) ? (ty == 0 ? TY - 1u : ty - 1u) : ty;
+        const uint32_t az = (j & 1) ? (tz == 0 ? TZ - 1u : tz - 1u) : tz;
+        w |= occ[((ax * TY + ay) * TZ + az) * 16u + (uint32_t)(i & 15)];
+      }
+    }
+    const uint64_t any = __ballot(w != 0u);
+    if (lane == 0) s_any = any != 0ull ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_any == 0u) return;
+  const float4 v = make_float4(value, value, value, value);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t c = threadIdx.x + 256u * (uint32_t)h;  // cell of the tile: (x, y, z) = c >> 6, (c >> 3) & 7, c & 7
+    const uint32_t x = tx * 8u + (c >> 6), y = ty * 8u + ((c >> 3) & 7u), z = tz * 8u + (c & 7u);
+    if (x < X && y < Y && z < Z) store_stream(grad + ((size_t)x * Y + y) * Z + z, v);
+  }
+}
+
 __global__ void k_fill_tail(float* __restrict__ dst, int64_t n, float value) {
   const int64_t i = threadIdx.x;
   if (i < n) dst[i] = value;
-}
-
-// Test hook: wave primitives on one wave of inputs (see pxl_debug_wave_ops).
-__global__ void k_debug_wave(const float* __restrict__ in, float* __restrict__ out) {
-  const int lane = threadIdx.x;
-  const float x = in[lane];
-  const float incl = wave_prefix_mul(x);
-  out[lane] = incl;
-  out[64 + lane] = wave_shift_up(incl, 1.f);
-  out[128 + lane] = wave_sum(x);
-  Affine4 f{in[64 + lane], in[128 + lane], in[192 + lane], in[256 + lane], in[320 + lane]};
-  wave_suffix_affine(f, lane);
-  out[192 + lane] = f.a0;
-  out[256 + lane] = f.a3;
-  out[320 + lane] = f.b;
-  out[384 + lane] = wave_shift_down(x, -1.f);
 }
 
 // Corner blocks -> gradient.  Entry E = (x, y, z) (4 float4) holds in slot s = 2bx + by the
@@ -851,9 +876,11 @@ __global__ __launch_bounds__(kReduceCells) void k_blocks_reduce(const float4* __
         acc.z += v[i][m].z;
         acc.w += v[i][m].w;
       }
-      if (active) {
+      // the cell's four source entries (bx, row i + by), any of which a step may have written
+      const uint32_t reach = (3u << i) | (3u << (NR + i));
+      if (active && (accumulate != PXL_REDUCE_OVERWRITE_REACHABLE || (written & reach) != 0u)) {
         const uint32_t c = base[1][i + 1] + z;
-        if (accumulate) {
+        if (accumulate == PXL_REDUCE_ACCUMULATE) {
           const float4 g = grad[c];
           acc = make_float4(g.x + acc.x, g.y + acc.y, g.z + acc.z, g.w + acc.w);
         }
@@ -1220,6 +1247,10 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
   if (!grid) return fail(PXL_E_INVALID, "grid descriptor is NULL");
   if ((rc = check_dims(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
   if (!blocks || !grad_grid) return fail(PXL_E_INVALID, "blocks and grad_grid are required");
+  if (accumulate != PXL_REDUCE_OVERWRITE && accumulate != PXL_REDUCE_ACCUMULATE &&
+      accumulate != PXL_REDUCE_OVERWRITE_REACHABLE)
+    return fail(PXL_E_INVALID, "unknown reduce mode %d", accumulate);
+  if (grid->occupancy && (rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
   const int64_t n = grid->dims[0] * grid->dims[1] * grid->dims[2];
   if (n > kMaxBlockCells)
     return fail(PXL_E_SIZE, "corner blocks index entry*4 + slot in 32 bits: at most %lld cells",
@@ -1231,7 +1262,7 @@ int pxl_blocks_reduce(const pxl_grid* grid, float* blocks, float* grad_grid, int
   const bool separate = clear == blocks;  // the pass reads `blocks`: clear them afterwards
   hipLaunchKernelGGL(pxl::k_blocks_reduce<pxl::kReduceRows>, dim3(n_wg), dim3(pxl::kReduceCells), 0, s,
                      reinterpret_cast<const float4*>(blocks), grid->occupancy, (int)grid->dims[0],
-                     (int)grid->dims[1], (int)grid->dims[2], accumulate ? 1 : 0,
+                     (int)grid->dims[1], (int)grid->dims[2], accumulate,
                      separate ? nullptr : reinterpret_cast<float4*>(clear),
                      reinterpret_cast<float4*>(grad_grid));
   if ((rc = launched("pxl_blocks_reduce"))) return rc;
@@ -1349,13 +1380,25 @@ int pxl_fill_f32(float* dst, int64_t n, float value, void* stream) {
   return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_fill_f32");
 }
 
-// Test hook, not part of the render path: runs the wave primitives on one wavefront.
-// in: 6*64 floats (x; affine a0, a1, a2, a3, b), out: 7*64 floats.
-int pxl_debug_wave_ops(const float* in, float* out, void* stream) {
-  if (!in || !out) return fail(PXL_E_INVALID, "in and out are required");
-  hipLaunchKernelGGL(pxl::k_debug_wave, dim3(1), dim3(64), 0, (hipStream_t)stream, in, out);
-  const hipError_t e = hipGetLastError();
-  return e == hipSuccess ? PXL_OK : hip_fail(e, "pxl_debug_wave_ops");
+int pxl_fill_reachable_f32(const pxl_grid* grid, int32_t interp, float* grad, float value, void* stream) {
+  int rc;
+  if (!grid) return fail(PXL_E_INVALID, "grid descriptor is NULL");
+  if ((rc = check_dims(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  if (!grid->occupancy || !grad) return fail(PXL_E_INVALID, "grid->occupancy and grad are required");
+  if (interp != PXL_INTERP_NEAREST && interp != PXL_INTERP_TRILINEAR) return fail(PXL_E_INVALID, "unknown interp %d", interp);
+  if ((rc = check_occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]))) return rc;
+  if (reinterpret_cast<uintptr_t>(grad) % 16 != 0) return fail(PXL_E_INVALID, "grad must be 16-byte aligned");
+  const int64_t tiles = occupancy_words(grid->dims[0], grid->dims[1], grid->dims[2]) / 16;
+  if (tiles > int64_t(0x7fffffff)) return fail(PXL_E_SIZE, "too many occupancy tiles for one launch");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
+  if (interp == PXL_INTERP_TRILINEAR)
+    hipLaunchKernelGGL(pxl::k_fill_reachable<true>, dim3((unsigned)tiles), dim3(256), 0, s, grid->occupancy, X, Y, Z,
+                       reinterpret_cast<float4*>(grad), value);
+  else
+    hipLaunchKernelGGL(pxl::k_fill_reachable<false>, dim3((unsigned)tiles), dim3(256), 0, s, grid->occupancy, X, Y,
+                       Z, reinterpret_cast<float4*>(grad), value);
+  return launched("pxl_fill_reachable_f32");
 }
 
 }  // extern "C"

@@ -258,11 +258,13 @@ class CornerBlockPair:
         self.active = 1 - self.active
 
 
-def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clear=None):
+def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clear=None, reachable_only=False):
     """grad_grid (+)= the gradient held in corner blocks (pxl_blocks_reduce).  clear: the
     blocks to zero for the next step -- default `blocks` itself (a second pass); another
     buffer is zeroed within the same pass; False: none (clear them with blocks_clear).
-    occupancy: the one the training steps used."""
+    occupancy: the one the training steps used.  reachable_only (with accumulate=False): write
+    only the cells a step with this occupancy can make nonzero, for a gradient buffer that holds
+    zeros everywhere else (zeroed once, used with one occupancy)."""
     L = _lib.lib()
     _check_blocks(grid, blocks, grad_grid)
     if clear is None:
@@ -272,7 +274,11 @@ def blocks_reduce(grid, blocks, grad_grid, occupancy=None, accumulate=True, clea
     elif clear is not blocks:
         _check_blocks(grid, clear, None)
     g = _grid_struct(grid, occupancy)
-    _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid), 1 if accumulate else 0,
+    if reachable_only and accumulate:
+        raise ValueError("reachable_only overwrites: use it with accumulate=False")
+    mode = (_lib.REDUCE_ACCUMULATE if accumulate else
+            _lib.REDUCE_OVERWRITE_REACHABLE if reachable_only else _lib.REDUCE_OVERWRITE)
+    _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid), mode,
                                    _lib.ptr(clear), _lib.stream_handle(grid.device)))
 
 
@@ -315,8 +321,9 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
 
     blocks (trilinear only): zeroed corner blocks (corner_blocks(grid)); the gradient is
     accumulated there and reduced into grad_grid before returning -- grad_grid is then
-    overwritten when overwrite=True (no zero-fill needed), else added to; `clear` as in
-    blocks_reduce.  grad_grid=None leaves it in the blocks for a later blocks_reduce
+    overwritten when overwrite=True (no zero-fill needed), else added to; overwrite="reachable"
+    writes only the cells a step with this occupancy can make nonzero (blocks_reduce's
+    reachable_only); `clear` as in blocks_reduce.  grad_grid=None leaves it in the blocks for a later blocks_reduce
     (several calls, one reduce).
     """
     L = _lib.lib()
@@ -370,8 +377,10 @@ def train_step(grid, grad_grid, *, cfg, origins=None, dirs=None, camera=None, ta
                 clear = blocks
             elif clear is not blocks:
                 _check_blocks(grid, clear, None)
+            mode = (_lib.REDUCE_ACCUMULATE if not overwrite else
+                    _lib.REDUCE_OVERWRITE_REACHABLE if overwrite == "reachable" else _lib.REDUCE_OVERWRITE)
             _lib.check(L.pxl_blocks_reduce(ctypes.byref(g), _lib.ptr(blocks), _lib.ptr(grad_grid),
-                                           0 if overwrite else 1, _lib.ptr(clear), stream))
+                                           mode, _lib.ptr(clear), stream))
     return ray_sq_err
 
 
@@ -413,6 +422,20 @@ def fill_(t, value=0.0):
     _lib.check(_lib.lib().pxl_fill_f32(_lib.ptr(t), t.numel(), ctypes.c_float(value),
                                        _lib.stream_handle(t.device)))
     return t
+
+
+def fill_reachable_(grad, occupancy, interp, value=0.0):
+    """The zero-fill of a training step with `occupancy` (pxl_fill_reachable_f32): grad (X,Y,Z,4)
+    float32 = value in every 8x8x8 tile holding a cell the step's scatter can write, other cells
+    untouched -- for a gradient buffer zeroed once and used with one occupancy."""
+    if grad.dtype != torch.float32 or grad.dim() != 4 or grad.shape[-1] != 4 or not grad.is_contiguous():
+        raise ValueError("grad must be a contiguous (X,Y,Z,4) float32 tensor")
+    if interp not in _INTERP:
+        raise ValueError(f"interp must be one of {sorted(_INTERP)}")
+    g = _grid_struct(grad, occupancy)
+    _lib.check(_lib.lib().pxl_fill_reachable_f32(ctypes.byref(g), _INTERP[interp], _lib.ptr(grad),
+                                                 ctypes.c_float(value), _lib.stream_handle(grad.device)))
+    return grad
 
 
 def occupancy_words(dims):

@@ -14,10 +14,30 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pxl.h")
 
 
-def header_functions():
-    src = open(HEADER).read()
+def header_functions(header=HEADER):
+    src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(pxl_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_probe_library_exports_its_header_and_nothing_of_the_render_path():
+    """libpxl_probe (measurement and test kernels) exports what include/pxl_probe.h declares; the
+    product library exports no probe or test hook."""
+    P = _lib.probe_lib()
+    names = header_functions(os.path.join(ROOT, "include", "pxl_probe.h"))
+    assert names == sorted(_lib.PROBE_SIGNATURES)
+    for n in names:
+        assert hasattr(P, n), n
+    assert not any("probe" in n or "debug" in n for n in header_functions())
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l and l.split()[-1].startswith("pxl_")}
+    assert exported == set(header_functions()), exported ^ set(header_functions())
+    assert P.pxl_probe_memory_rate(None, 16, 0, 4, 1, 1, 0, None, None) == _lib.PXL_E_INVALID
+    assert P.pxl_probe_memory_rate(ctypes.c_void_p(0x1000), 16, 0, 3, 1, 1, 0, None, None) == _lib.PXL_E_INVALID
+    assert P.pxl_probe_memory_rate(ctypes.c_void_p(0x1000), 4, 0, 4, 1, 1, 1, None, None) == _lib.PXL_E_INVALID
+    assert P.pxl_probe_memory_rate(ctypes.c_void_p(0x1000), 16, 2, 4, 1, 1, 0, None, None) == _lib.PXL_E_INVALID
+    assert P.pxl_probe_memory_rate(ctypes.c_void_p(0x1000), 16, 0, 4, 0, 1, 0, None, None) == _lib.PXL_OK
 
 
 def test_library_exports_every_declared_symbol():
@@ -257,6 +277,17 @@ def test_scaled_sum_validation():
     assert L.pxl_scaled_sum(ctypes.c_void_p(0x3004), 5, 1.0, out, work, None) == _lib.PXL_E_INVALID
     assert "aligned" in _lib.last_error()
     assert L.pxl_scaled_sum(ctypes.c_void_p(0x3000), 5, 1.0, out, ctypes.c_void_p(0x1008), None) == _lib.PXL_E_INVALID
+    # reachable-only fill and reduce modes: occupancy required, unknown modes rejected
+    g, m, r = _valid()
+    assert L.pxl_fill_reachable_f32(ctypes.byref(g), 0, ctypes.c_void_p(0x3000), ctypes.c_float(0.0),
+                                    None) == _lib.PXL_E_INVALID  # no occupancy
+    g.occupancy = 0x7000
+    assert L.pxl_fill_reachable_f32(ctypes.byref(g), 5, ctypes.c_void_p(0x3000), ctypes.c_float(0.0),
+                                    None) == _lib.PXL_E_INVALID  # unknown interp
+    assert L.pxl_fill_reachable_f32(ctypes.byref(g), 0, ctypes.c_void_p(0x3004), ctypes.c_float(0.0),
+                                    None) == _lib.PXL_E_INVALID  # misaligned
+    assert L.pxl_blocks_reduce(ctypes.byref(g), ctypes.c_void_p(0x3000), ctypes.c_void_p(0x4000), 3, None,
+                               None) == _lib.PXL_E_INVALID and "mode" in _lib.last_error()
     # the allocation-free variant validates the same way, before it looks up its scratch
     assert L.pxl_scaled_sum_noalloc(ctypes.c_void_p(0x3000), -1, 1.0, out, None) == _lib.PXL_E_INVALID
     assert L.pxl_scaled_sum_noalloc(None, 5, 1.0, out, None) == _lib.PXL_E_INVALID

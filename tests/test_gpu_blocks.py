@@ -276,3 +276,87 @@ def test_blocks_odd_shape_equal_direct(occupied):
     ok, info = grad_close(g2, g1, rel=1e-5)
     say("  blocks vs direct", info)
     assert ok, info
+
+
+def _tiles_of(cells):
+    """Per 8x8x8 tile of a (X,Y,Z) bool grid: any cell set (padded to whole tiles)."""
+    X, Y, Z = cells.shape
+    TX, TY, TZ = (X + 7) // 8, (Y + 7) // 8, (Z + 7) // 8
+    pad = torch.zeros(TX * 8, TY * 8, TZ * 8, dtype=torch.bool)
+    pad[:X, :Y, :Z] = cells
+    return pad.reshape(TX, 8, TY, 8, TZ, 8).any(5).any(3).any(1)
+
+
+def _cells_of_tiles(tiles, dims):
+    X, Y, Z = dims
+    full = tiles.repeat_interleave(8, 0).repeat_interleave(8, 1).repeat_interleave(8, 2)
+    return full[:X, :Y, :Z]
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_fill_reachable_covers_every_cell_a_step_writes(interp):
+    """pxl_fill_reachable_f32 fills whole 8^3 tiles: those with an occupied cell (nearest), or
+    with one in the tile or a lower neighbour tile, wrapped (trilinear: corners reach one cell up);
+    that covers every cell the step's scatter can write, and nothing else is touched."""
+    from plenoxels_amd import fill_reachable_
+    say("fill reachable", interp)
+    dims = (37, 34, 45)  # partial tiles on every axis
+    occ = torch.zeros(dims, dtype=torch.bool)
+    occ[3:14, 9:20, 5:17] = torch.rand(11, 11, 12, generator=torch.Generator().manual_seed(70)) < 0.02
+    occ[36, 33, 44] = True  # a floor cell at the top corner: its ceil corners wrap to index 0
+    pts = pack_occupancy(occ.to(DEV))
+    bits = occupancy_dilate(pts, dims) if interp == "trilinear" else pts
+    g = torch.full(dims + (4,), 7.0, device=DEV)
+    fill_reachable_(g, bits, interp)
+    filled = (g.cpu() == 0.0).all(-1)
+    assert bool(((g.cpu() == 0.0) | (g.cpu() == 7.0)).all())
+    cells = plain.dilate_cells(occ) if interp == "trilinear" else occ  # floor cells a step keeps
+    tiles = _tiles_of(cells)
+    if interp == "trilinear":
+        t = tiles.clone()
+        for a in (0, 1):
+            for b in (0, 1):
+                for c in (0, 1):
+                    t |= torch.roll(tiles, shifts=(a, b, c), dims=(0, 1, 2))
+        tiles = t
+    assert torch.equal(filled, _cells_of_tiles(tiles, dims))
+    # every cell the scatter can write: the kept cell itself, or the 8 corners of a kept floor cell
+    writes = cells.clone()
+    if interp == "trilinear":
+        for a in (0, 1):
+            for b in (0, 1):
+                for c in (0, 1):
+                    writes |= torch.roll(cells, shifts=(a, b, c), dims=(0, 1, 2))
+    assert bool(filled[writes].all())
+    assert int(filled.sum()) < filled.numel()  # something was left alone
+
+
+def test_reduce_reachable_only_equals_the_dense_reduce_where_it_writes():
+    """PXL_REDUCE_OVERWRITE_REACHABLE writes the dense reduce's values at every cell a source entry
+    of a set occupancy bit reaches, leaves every other cell untouched, and the dense result is zero
+    at those other cells -- so on a buffer zeroed once the two modes give the same gradient."""
+    say("reduce reachable only")
+    grid, o, d, gmin, pd = small_scene(72, X=22, R=900)
+    dims = tuple(grid.shape[:3])
+    occ = torch.rand(dims, generator=torch.Generator().manual_seed(73)) < 0.05
+    bits = occupancy_dilate(pack_occupancy(occ.to(DEV)), dims)
+    cells = plain.dilate_cells(occ)
+    tgt = torch.rand(o.shape[0], 4, generator=torch.Generator().manual_seed(74))
+    cfg = MarchConfig(140, 0.03, tuple(gmin.tolist()), pd, "trilinear")
+    gd = grid.to(DEV)
+    blocks = corner_blocks(gd)
+    train_step(gd, None, cfg=cfg, origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), occupancy=bits,
+               blocks=blocks)
+    dense = torch.full(grid.shape, 5.0, device=DEV)
+    blocks_reduce(gd, blocks, dense, occupancy=bits, accumulate=False, clear=False)
+    sparse = torch.full(grid.shape, 7.0, device=DEV)
+    blocks_reduce(gd, blocks, sparse, occupancy=bits, accumulate=False, clear=blocks, reachable_only=True)
+    assert int(torch.count_nonzero(blocks)) == 0
+    # entry (x, y, z) may hold sums when the floor cell (x, y, z) or (x, y, z - 1) is kept; cell
+    # (x, y, z) reads entries (x - 1 + bx, y - 1 + by, z)
+    written = cells | torch.roll(cells, 1, 2)
+    reach = written | torch.roll(written, 1, 0) | torch.roll(written, 1, 1) | torch.roll(written, (1, 1), (0, 1))
+    dense, sparse = dense.cpu(), sparse.cpu()
+    assert torch.equal(sparse[reach], dense[reach])
+    assert bool((sparse[~reach] == 7.0).all()) and bool((dense[~reach] == 0.0).all())
+    assert bool((dense[reach] != 0).any())

@@ -31,7 +31,7 @@ def test_wave_primitives():
     aff = torch.rand(5, 64, generator=g)
     inp = torch.cat([x, aff.reshape(-1)]).to(DEV)
     out = torch.empty(7 * 64, device=DEV)
-    _lib.check(_lib.lib().pxl_debug_wave_ops(_lib.ptr(inp), _lib.ptr(out), _lib.stream_handle(DEV)))
+    _lib.check(_lib.probe_lib().pxl_probe_wave_ops(_lib.ptr(inp), _lib.ptr(out), _lib.stream_handle(DEV)))
     out = out.cpu().double().reshape(7, 64)
     xd = x.double()
     incl = torch.cumprod(xd, 0)
