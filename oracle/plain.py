@@ -371,14 +371,61 @@ def tv_loss(input_tensor):
     return torch.sqrt(gx.sum() + gy.sum() + gz.sum())
 
 
+def _chunked_step(cells, o, d, pix, N, S, delta_step, near, gmin, pd, div_mode, tv, beta, chunk, opt):
+    """One step of reference_fit over ray chunks (see its chunk_rays): mse_loss's mean and the
+    beta term's mean as sums over chunks divided by the full batch's counts."""
+    leaf = cells.detach().requires_grad_(True)
+    pix = pix.to(leaf.dtype)
+    closs = 0.0
+    bl = 0.0
+    for s in range(0, N, chunk):
+        e = min(N, s + chunk)
+        p = sample_positions(o[s:e], d[s:e], S, delta_step, near)
+        n = normalize_samples(p, gmin, pd, div_mode)
+        nearest, _ = nearest_values(n, leaf)
+        nearest = nearest.reshape(1, e - s, S, 4)
+        col = compute_alpha_weighted_pixels(nearest).reshape(e - s, 4)
+        part = ((col - pix[s:e]) ** 2).sum() / (4 * N)
+        total = part
+        closs += float(part.detach())
+        if beta > 0:
+            a = nearest[..., -1]
+            bpart = beta * (torch.log(a + 1e-4) - torch.log(1 - a + 1e-4)).sum() / (N * S)
+            bl += float(bpart.detach())
+            total = total + bpart
+        total.backward()
+    rec = {"closs": closs}
+    if tv > 0:
+        t = tv * tv_loss(leaf)
+        t.backward()
+        rec["tvloss"] = float(t.detach())
+    if beta > 0:
+        rec["betaloss"] = bl
+    opt.zero_grad()
+    cells.backward(leaf.grad)  # through the pooling (or straight into the leaf grid)
+    opt.step()
+    return rec
+
+
 def reference_fit(imgs, c2w, camera_angle_x, *, gridsize, points_distance, num_samples, delta_step, lr, tv,
                   beta, uniforms, progressive_growing=True, div_mode="divide", start_step=0, init_grid=None,
-                  norm_mode="match"):
+                  norm_mode="match", near=0.0, chunk_rays=None, accum_dtype=None):
     """scripts/train.py:79-184 op for op on given per-step uniforms (list of (B,R,2)).
 
-    Returns (grid, grid_grad_abs_sum, per-step loss dicts).  Rays: camera-major, R per camera.
+    Returns (grid, grid_grad_abs_sum, per-step loss dicts).  Rays: camera-major, R per camera;
+    or, for an entry (cam_ids (N,), uv (N,2)), N rays from the given cameras (an extension for
+    batch sizes the camera count does not divide: the same chain on one (1, N) ray batch).
     div_mode / norm_mode: the division and 3-norm semantics ("match": the norm of the device
     whose division div_mode names, see device_norm_mode; None: torch's on imgs' device).
+    near: first sample at near + delta (extension; 0 is the reference).
+    chunk_rays: for batches too large for one autograd graph (torch's GPU kernels refuse launch
+    shapes beyond a few hundred million samples), the step's loss terms are formed and
+    back-propagated ray chunk by ray chunk into the rendered grid cut from the graph, whose
+    gradient then flows once through the pooling: the same gradient up to summation order.
+    accum_dtype=torch.float64 (with chunk_rays): the FP64 evaluation of the same step -- sample
+    placement and voxel choice stay float32 as in the reference, while the pooled grid, the
+    gathered values, the compositing, the losses and every gradient sum are float64; the FP32
+    parameter's gradient is that sum rounded once.
     """
     if norm_mode == "match":
         norm_mode = device_norm_mode(div_mode)
@@ -399,15 +446,30 @@ def reference_fit(imgs, c2w, camera_angle_x, *, gridsize, points_distance, num_s
             gg = grid_grid_original[start::stride, start::stride, start::stride]
             gi = gg.reshape(-1, 3)
             pd = points_distance * stride
-            cells = average_pool3d_grid(grid_full, receptive_field_size=k, stride=stride)
+            src = grid_full if accum_dtype is None else grid_full.to(accum_dtype)
+            cells = average_pool3d_grid(src, receptive_field_size=k, stride=stride)
         else:
-            cells = grid_full
+            cells = grid_full if accum_dtype is None else grid_full.to(accum_dtype)
             gi = grid_grid_original.reshape(-1, 3)
             pd = points_distance
-        B, R = uv.shape[0], uv.shape[1]
-        d, pix = generate_rays_batched(imgs, R, c2w, camera_angle_x, uv, norm_mode)
-        o = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
-        p = sample_positions(o, d, num_samples, delta_step)
+        if isinstance(uv, tuple):  # explicit (cam_ids, uv) rays
+            cam, uvr = uv
+            cam = cam.to(imgs.device).long()
+            B, R = 1, cam.shape[0]
+            o, d, pix = camera_rays(imgs, c2w, camera_angle_x, cam, uvr[:, 0].contiguous(), uvr[:, 1].contiguous(),
+                                    norm_mode)
+        else:
+            B, R = uv.shape[0], uv.shape[1]
+            d, pix = generate_rays_batched(imgs, R, c2w, camera_angle_x, uv, norm_mode)
+            o = torch.repeat_interleave(c2w[:, :3, 3], R, 0)
+        N = B * R
+        if chunk_rays is not None and (N > chunk_rays or accum_dtype is not None):
+            rec = _chunked_step(cells, o, d, pix, N, num_samples, delta_step, near, grid_min(gi), pd, div_mode,
+                                tv if k < 19 else 0.0, beta, chunk_rays, opt)
+            grad_abs += torch.abs(grid_full.grad)
+            hist.append(rec)
+            continue
+        p = sample_positions(o, d, num_samples, delta_step, near)
         n = normalize_samples(p, grid_min(gi), pd, div_mode)
         nearest, mask = nearest_values(n, cells)
         nearest = nearest.reshape(B, R, num_samples, 4)

@@ -10,7 +10,8 @@ import os
 import torch  # noqa: F401  (loads the HIP runtime libpxl binds to)
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
-LIB_PATH = os.path.join(LIB_DIR, "libpxl.so")
+# PXL_LIBRARY names another build of the same library (e.g. the sanitizer build the ABI tests load)
+LIB_PATH = os.environ.get("PXL_LIBRARY") or os.path.join(LIB_DIR, "libpxl.so")
 PROBE_LIB_PATH = os.path.join(LIB_DIR, "libpxl_probe.so")
 
 PXL_OK = 0

@@ -66,12 +66,16 @@ class Trainer:
     is >= 0 (pxl_alpha_occupancy): the samples it skips are exactly zero in the reference chain
     (clamped alpha 0, zero clamp gradient), so finite results are unchanged while the empty
     space the training drives below zero costs nothing.
+    total_rays (extension): draw that many rays per step, each from a camera chosen uniformly
+    at random, instead of number_of_rays per camera (the reference's scheme) -- for batch sizes
+    the camera count does not divide, e.g. 2^19 rays over 100 views; uniforms_fn(step) then
+    returns (cam_ids (N,) int32, uv (N,2)).
     """
 
     def __init__(self, imgs, transform_matricies, camera_angle_x, *, gridsize, points_distance,
                  number_of_rays, num_samples, delta_step, lr, tv, beta, even_spread=False, device="cuda",
                  progressive_growing=True, interp="nearest", storage_dtype=torch.float32, uniforms_fn=None,
-                 init_grid=None, near=0.0, t_stop=0.0, corner_blocks=False, skip_empty=True):
+                 init_grid=None, near=0.0, t_stop=0.0, corner_blocks=False, skip_empty=True, total_rays=None):
         self.dev = torch.device(device)
         self.imgs = imgs.to(self.dev).float().contiguous()
         self.c2w = transform_matricies.to(self.dev).float().contiguous()
@@ -107,7 +111,11 @@ class Trainer:
         self.grad_abs = torch.zeros_like(self.grid)
         self.render_copy = None if storage_dtype == torch.float32 else self.grid.to(storage_dtype)
         self.step_count = 0
-        n = self.B * self.R
+        self.total_rays = None if total_rays is None else int(total_rays)
+        if self.total_rays is not None and self.even_spread:
+            raise ValueError("total_rays draws random rays; it does not combine with even_spread")
+        n = self.B * self.R if self.total_rays is None else self.total_rays
+        self.n_rays = n
         self.cam_ids = torch.repeat_interleave(torch.arange(self.B, dtype=torch.int32, device=self.dev), self.R)
         self.sq = torch.empty(n, device=self.dev)
         self.loss_mean = LossMean(self.dev)
@@ -126,6 +134,16 @@ class Trainer:
         if self.even_spread:
             return self.even_uv
         return torch.rand(self.B, self.R, 2, device=self.dev)
+
+    def _rays(self, i):
+        """(cam_ids (N,) int32, uv (N,2) float32) of step i."""
+        if self.total_rays is None:
+            return self.cam_ids, self._uniforms(i).reshape(self.B * self.R, 2).contiguous()
+        if self.uniforms_fn is not None:
+            cam, uv = self.uniforms_fn(i)
+            return cam.to(self.dev).to(torch.int32).contiguous(), uv.to(self.dev).float().contiguous()
+        cam = torch.randint(0, self.B, (self.total_rays,), device=self.dev, dtype=torch.int32)
+        return cam, torch.rand(self.total_rays, 2, device=self.dev)
 
     def step(self, i):
         """One training step; returns device scalars {"closs", "tvloss", "betaloss", "loss"}."""
@@ -153,10 +171,10 @@ class Trainer:
             if not self.use_blocks and not self._grad_zero:
                 fill_(grad, 0.0)
             self._grad_zero = False
-        uv = self._uniforms(i).reshape(self.B * self.R, 2).contiguous()
-        cam = CameraRays(self.cam_ids, uv, self.c2w, self.theta, self.imgs)
+        cam_ids, uv = self._rays(i)
+        cam = CameraRays(cam_ids, uv, self.c2w, self.theta, self.imgs)
         cfg = MarchConfig(self.S, self.delta, gmin, pd, self.interp, self.near, self.t_stop)
-        n_rays = self.B * self.R
+        n_rays = self.n_rays
         pair = None
         if self.use_blocks:
             key = tuple(grid.shape[:3])

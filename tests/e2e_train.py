@@ -35,7 +35,7 @@ if ROOT not in sys.path:
 
 from oracle import cref, plain  # noqa: E402
 from plenoxels_amd import CameraRays, MarchConfig, scenes  # noqa: E402
-from plenoxels_amd.render import render_camera  # noqa: E402
+from plenoxels_amd.render import render_camera, render_rays  # noqa: E402
 from plenoxels_amd.train import Trainer  # noqa: E402
 
 
@@ -175,6 +175,14 @@ def main(argv=None):
     imgs_dev = imgs_train.to(dev)
     held_dev = imgs_held.to(dev)
     init = torch.tensor([0.5, 0.5, 0.5, 0.1])
+    # a fixed batch of training rays on which the final loss is evaluated, by the fused kernel and
+    # by the FP64 oracle on the trained FP32 grid
+    ge = torch.Generator().manual_seed(11)
+    n_ev = min(8192, a.rays)
+    ev_cam = torch.randint(0, a.views, (n_ev,), generator=ge)
+    ev_uv = torch.rand(n_ev, 2, generator=ge)
+    ev_o, ev_d, ev_t = plain.camera_rays(imgs_train, c2w_train, theta, ev_cam, ev_uv[:, 0].contiguous(),
+                                         ev_uv[:, 1].contiguous(), norm_mode="gpu")
     R_per_cam = a.rays // a.views
     res["fused"] = {}
     for name in a.dtypes.split(","):
@@ -186,6 +194,9 @@ def main(argv=None):
                      skip_empty=bool(a.skip_empty))
         rgrid = lambda: tr.grid if tr.render_copy is None else tr.render_copy  # noqa: E731
         p0 = psnr(render_views_fused(rgrid(), c2w_held, theta, a.res, cfg_full, dev)[..., :3], held_dev[..., :3])
+        # every dtype trains on the same ray draws (torch.rand on the device, re-seeded), so the
+        # runs differ only in the storage precision
+        torch.manual_seed(1234)
         torch.cuda.synchronize()
         t0 = time.time()
         hist = []
@@ -199,12 +210,21 @@ def main(argv=None):
         wall = time.time() - t0
         held = render_views_fused(rgrid(), c2w_held, theta, a.res, cfg_full, dev)
         train_v = render_views_fused(rgrid(), c2w_train[:5], theta, a.res, cfg_full, dev)
+        with torch.no_grad():
+            out_ev = render_rays(rgrid(), ev_o.to(dev), ev_d.to(dev), num_samples=S, delta_step=delta, grid_min=gmin,
+                                 points_distance=pd, interp="trilinear", near=a.near, t_stop=a.t_stop).cpu()
+        loss_fused = float(((out_ev.double() - ev_t.double()) ** 2).mean())
+        rgba64 = cref.render_parallel(tr.grid.double().cpu().numpy(), ev_o.numpy(), ev_d.numpy(), threads=a.threads,
+                                      chunk=4096, **kw)
+        loss64 = float(np.mean((rgba64 - ev_t.double().numpy()) ** 2))
         r = {"wall_s": round(wall, 2), "ms_per_iter": round(wall / a.iters * 1e3, 3),
              "rays_per_s": round(a.iters * R_per_cam * a.views / wall, 1), "final_loss": hist[-1][1] if hist else None,
              "heldout_psnr_rgb_init": round(p0, 3),
              "heldout_psnr_rgb": round(psnr(held[..., :3], held_dev[..., :3]), 3),
              "heldout_psnr_rgba": round(psnr(held, held_dev), 3),
              "train5_psnr_rgb": round(psnr(train_v[..., :3], imgs_dev[:5, ..., :3]), 3),
+             "final_loss_eval": {"rays": n_ev, "fused": loss_fused, "fp64_of_fp32_master": loss64,
+                                 "rel_err": abs(loss_fused - loss64) / loss64},
              "loss_history": hist}
         res["fused"][name] = r
         log(f"{name}: " + json.dumps({k: v for k, v in r.items() if k != "loss_history"}))

@@ -167,21 +167,31 @@ def test_sparse256_full_batch_vs_plain_gpu(interp):
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 def test_sparse256_half_storage_error_report(dtype):
+    """Half-precision storage on the benchmark scene (4096 rays, trilinear), against the FP64 C
+    oracle on the FP32 grid: per-result errors of the render, the gradient and the step's loss.
+    The PSNR bar is taken where PSNR means something: the targets are the FP64 oracle's render
+    plus seeded noise at 35 dB (a well-trained reconstruction), so the half-precision render's
+    PSNR against them may drop by at most 0.1 dB from the FP32 render's.  The render error itself
+    is bounded by the storage rounding: a pixel is a convex combination of clamped cells, so its
+    error stays within a small multiple of the format's half-ulp at 1 (2^-9 BF16, 2^-12 FP16)."""
     say("sparse256 precision report", dtype)
     w = scenes.make_workload("sparse256")
     n = 4096
     o, d = _rays(w, n)
-    tgt = scenes.random_targets(n)
     occ_cells = plain.dilate_cells(w.occupancy)
     bits_np = plain.occupancy_to_bits(occ_cells).numpy()
     kw = dict(num_samples=w.num_samples, delta_step=w.delta_step, gmin=w.grid_min, points_distance=w.points_distance,
               interp="trilinear", near=w.near, div_mode="reciprocal")
     rgba64, _, _ = cref.render(w.grid.numpy(), o.numpy(), d.numpy(), occupancy_bits=bits_np, **kw)
-    up = 2 * (rgba64 - tgt.double().numpy()) / (4 * n)
+    noise = np.random.default_rng(9).normal(size=rgba64.shape) * 10 ** (-35 / 20)
+    tgt64 = np.clip(rgba64 + noise, 0.0, 1.0)
+    tgt = torch.from_numpy(tgt64).float()
+    tgt64 = tgt.double().numpy()
+    up = 2 * (rgba64 - tgt64) / (4 * n)
     _, g64, _ = cref.render(w.grid.numpy(), o.numpy(), d.numpy(), occupancy_bits=bits_np, grad_rgba=up, **kw)
     bits = occupancy_dilate(pack_occupancy(w.occupancy.to(DEV)), w.dims)
     cfg = MarchConfig(w.num_samples, w.delta_step, w.grid_min, w.points_distance, "trilinear", w.near)
-    loss64 = float(np.mean((rgba64 - tgt.double().numpy()) ** 2))
+    loss64 = float(np.mean((rgba64 - tgt64) ** 2))
     loss_mean = LossMean(DEV)
     res = {}
     for name, dt in (("f32", torch.float32), ("half", dtype)):
@@ -194,11 +204,12 @@ def test_sparse256_half_storage_error_report(dtype):
         e = np.abs(out.cpu().double().numpy() - rgba64)
         gk = gg.cpu().double().numpy()
         gerr = np.abs(gk - g64).max() / np.abs(g64).max()
-        psnr = 10 * np.log10(1.0 / np.mean((out.cpu().double().numpy() - tgt.double().numpy()) ** 2))
+        psnr = 10 * np.log10(1.0 / np.mean((out.cpu().double().numpy() - tgt64) ** 2))
         res[name] = {"rgba_max_abs_err": float(e.max()), "rgba_mean_abs_err": float(e.mean()),
                      "grad_rel_err": float(gerr), "psnr_vs_targets_db": float(psnr),
                      "loss": loss, "loss_rel_err_vs_fp64": abs(loss - loss64) / loss64}
-    psnr64 = 10 * np.log10(1.0 / np.mean((rgba64 - tgt.double().numpy()) ** 2))
+    psnr64 = 10 * np.log10(1.0 / np.mean((rgba64 - tgt64) ** 2))
+    res["targets"] = "FP64 render + N(0, 10^(-35/20)) noise, clipped to [0, 1]"
     res["fp64_psnr_vs_targets_db"] = float(psnr64)
     res["fp64_loss"] = loss64
     res["psnr_delta_half_vs_f32_db"] = res["f32"]["psnr_vs_targets_db"] - res["half"]["psnr_vs_targets_db"]
@@ -206,7 +217,11 @@ def test_sparse256_half_storage_error_report(dtype):
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, f"precision_sparse256_{str(dtype).split('.')[-1]}.json"), "w") as f:
         json.dump(res, f, indent=1)
+    assert 30.0 <= psnr64 <= 40.0  # the bar is taken at a realistic reconstruction quality
     assert res["f32"]["rgba_max_abs_err"] <= 1e-5
     assert res["f32"]["grad_rel_err"] <= 1e-4
     assert res["f32"]["loss_rel_err_vs_fp64"] <= 1e-5
+    half_ulp = 2.0 ** -9 if dtype == torch.bfloat16 else 2.0 ** -12
+    assert res["half"]["rgba_max_abs_err"] <= 4 * half_ulp
+    assert res["half"]["rgba_mean_abs_err"] <= half_ulp / 4
     assert abs(res["psnr_delta_half_vs_f32_db"]) <= 0.1

@@ -1,7 +1,8 @@
 """End-to-end optimisation at reduced size (tests/e2e_train.py at 64^3, 8 training views and
 2 held-out views of 64x64, 4096 rays x 128 samples per iteration, 150 iterations, no
 progressive growing): the fused forward of the ground-truth grid matches the FP64 oracle's
-images, training raises the held-out PSNR, and BF16 storage trains to the FP32 result."""
+images, training raises the held-out PSNR, the trained grid's loss matches an FP64 evaluation,
+and BF16 storage trains to within 0.1 dB of the FP32 result's held-out PSNR."""
 import os
 import sys
 
@@ -25,6 +26,13 @@ def test_e2e_small_bf16_and_fp32(tmp_path):
     assert par["max_abs_err"] <= 1e-4, par
     for name in ("bfloat16", "float32"):
         f = r["fused"][name]
+        say(f"  {name}: held-out PSNR {f['heldout_psnr_rgb']} dB (init {f['heldout_psnr_rgb_init']}), "
+            f"final loss {f['final_loss_eval']}")
         assert f["heldout_psnr_rgb"] >= f["heldout_psnr_rgb_init"] + 3.0, f
         assert f["loss_history"][-1][1] < f["loss_history"][0][1], f
-    assert abs(r["heldout_psnr_delta_bf16_vs_fp32_db"]) <= 0.5, r["heldout_psnr_delta_bf16_vs_fp32_db"]
+    # the trained grid's loss by the fused kernel vs the FP64 oracle on the FP32 master: FP32
+    # storage to the forward bar; BF16 storage carries the rounding of the render copy
+    assert r["fused"]["float32"]["final_loss_eval"]["rel_err"] <= 1e-4
+    assert r["fused"]["bfloat16"]["final_loss_eval"]["rel_err"] <= 1e-2
+    say("  held-out PSNR delta bf16 - fp32:", r["heldout_psnr_delta_bf16_vs_fp32_db"], "dB")
+    assert abs(r["heldout_psnr_delta_bf16_vs_fp32_db"]) <= 0.1, r["heldout_psnr_delta_bf16_vs_fp32_db"]

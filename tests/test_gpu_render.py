@@ -411,3 +411,49 @@ def test_empty_ragged_and_missing_batches(interp):
             ok, info = grad_close(gb.cpu(), gwant)
             assert ok, info
             assert int(torch.count_nonzero(blocks)) == 0
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_early_termination_boundary_is_exact(interp):
+    """Early termination keeps a sample whose transmittance equals t_stop and drops the first one
+    below it.  Every cell has alpha exactly 0.5 and the rays run along +z through lattice points
+    (delta = points_distance, all coordinates short binary fractions), so sample i inside the grid
+    sees T = 2^-i exactly in any summation order: with t_stop = 2^-6 seven samples contribute
+    (A = 1 - 2^-7 exactly), with the next float above 2^-6 six do (A = 1 - 2^-6); RGBA equals the
+    oracle's and the gradient its autograd gradient."""
+    say("early termination boundary", interp)
+    X, pd = 12, 0.25
+    g = torch.Generator().manual_seed(90)
+    grid = torch.rand(X, X, X, 4, generator=g) * 0.8 + 0.1
+    grid[..., 3] = 0.5
+    gi, _, _, _ = plain.generate_grid(X, X, X, points_distance=pd)
+    gmin = plain.grid_min(gi)
+    R = 96
+    ij = torch.randint(0, X, (R, 2), generator=g).float()
+    o = torch.empty(R, 3)
+    o[:, :2] = gmin[:2] + ij * pd
+    o[:, 2] = gmin[2] - 3 * pd  # sample 3 is the first inside the grid
+    d = torch.zeros(R, 3)
+    d[:, 2] = 1.0
+    tgt = torch.rand(R, 4, generator=g)
+    S = 30
+    for t_stop, n_kept in ((2.0 ** -6, 7), (float(np.nextafter(np.float32(2.0 ** -6), np.float32(1))), 6)):
+        kw = dict(num_samples=S, delta_step=pd, points_distance=pd, interp=interp, t_stop=t_stop)
+        want, gwant = _oracle(grid, o, d, tgt, gmin=gmin, div_mode="reciprocal", **kw)
+        assert torch.equal(want[:, 3], torch.full((R,), 1 - 2.0 ** -n_kept))
+        gd = grid.to(DEV).requires_grad_(True)
+        out = render_rays(gd, o.to(DEV), d.to(DEV), grid_min=gmin.tolist(), **kw)
+        ((out - tgt.to(DEV)) ** 2).mean().backward()
+        out = out.detach().cpu()
+        assert torch.equal(out[:, 3], want[:, 3]), (t_stop, out[:3, 3])
+        assert (out - want).abs().max().item() <= 1e-6
+        ok, info = grad_close(gd.grad.cpu(), gwant)
+        assert ok, (t_stop, info)
+        # the fused train step cuts at the same sample
+        gg = torch.zeros(grid.shape, device=DEV)
+        so = torch.empty(R, 4, device=DEV)
+        train_step(grid.to(DEV), gg, cfg=MarchConfig(S, pd, tuple(gmin.tolist()), pd, interp, 0.0, t_stop),
+                   origins=o.to(DEV), dirs=d.to(DEV), targets=tgt.to(DEV), out=so)
+        assert torch.equal(so.cpu()[:, 3], want[:, 3])
+        ok, info = grad_close(gg.cpu(), gwant)
+        assert ok, (t_stop, info)

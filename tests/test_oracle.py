@@ -446,3 +446,27 @@ def test_norm3_orders_are_within_an_ulp_and_distinct():
     # "gpu" is the x, z pair summed first: swapping y and z changes it, swapping x and z does not
     assert torch.equal(plain.norm3(v[:, [2, 1, 0]], "gpu"), plain.norm3(v, "gpu"))
     assert not torch.equal(plain.norm3(v[:, [0, 2, 1]], "gpu"), plain.norm3(v, "gpu"))
+
+
+@pytest.mark.parametrize("start,progressive", [(0, False), (185, True), (229, True)])
+def test_reference_fit_chunked_equals_one_graph(start, progressive):
+    """reference_fit's ray-chunked step (for batches too large for one autograd graph) gives the
+    one-graph step's losses, |grad| and updated grid up to summation order: full-grid steps,
+    pooled steps, the TV and beta terms."""
+    from util import random_cameras
+    B, H, R = 3, 16, 40
+    c2w = random_cameras(B, 5, radius=2.0)
+    imgs = torch.rand(B, H, H, 4, generator=torch.Generator().manual_seed(1))
+    g = torch.Generator().manual_seed(2)
+    uni = [torch.rand(B, R, 2, generator=g) for _ in range(3)]
+    init = torch.rand(24, 24, 24, 4, generator=torch.Generator().manual_seed(3)) * 0.6 - 0.1
+    kw = dict(gridsize=(24, 24, 24), points_distance=0.06, num_samples=40, delta_step=0.05, lr=0.01, tv=1e-3,
+              beta=5e-3, uniforms=uni, start_step=start, init_grid=init, progressive_growing=progressive)
+    ga, aa, ha = plain.reference_fit(imgs, c2w, 0.69, **kw)
+    gb, ab, hb = plain.reference_fit(imgs, c2w, 0.69, chunk_rays=17, **kw)
+    assert (ga - gb).abs().max().item() <= 1e-6
+    assert (aa - ab).abs().max().item() <= 1e-6 * aa.abs().max().item()
+    for x, y in zip(ha, hb):
+        assert set(x) == set(y)
+        for k in x:
+            assert abs(x[k] - y[k]) <= 1e-6 * abs(y[k])
