@@ -259,9 +259,9 @@ int64_t pxl_occupancy_words(int64_t X, int64_t Y, int64_t Z);
 int pxl_fill_f32(float* dst, int64_t n, float value, void* stream);
 
 /* The gradient zero-fill of a training step with grid->occupancy: grad (X,Y,Z,4) f32 = value
- * in every 8x8x8 tile that holds a cell the step's scatter can write -- nearest: a tile with an
- * occupied cell; trilinear (corners one cell above the floor cell, wrapped): a tile whose own
- * bits or those of a lower neighbour tile (x-1, y-1, z-1, wrapped) are set.  Other cells are not
+ * at every cell the step's scatter can write -- nearest: each occupied cell; trilinear (corners
+ * one cell above the floor cell, wrapped): every cell of each 8x8x8 tile whose own bits or
+ * those of a lower neighbour tile (x-1, y-1, z-1, wrapped) are set.  Other cells are not
  * touched: a gradient buffer zeroed once and used with one occupancy then holds exactly what a
  * dense zero-fill would give it.  grid->data is not read; grid->occupancy is required. */
 int pxl_fill_reachable_f32(const pxl_grid* grid, int32_t interp, float* grad, float value, void* stream);

@@ -701,44 +701,50 @@ __global__ void k_fill(float4* __restrict__ dst, int64_t n4, float value) {
     dst[i] = v;
 }
 
-// Fill of the 8x8x8 tiles a scatter can reach (pxl_fill_reachable_f32): one workgroup per tile.
-// Lanes 0..15 of wave 0 read the tile's own occupancy words (TRI: also those of the seven tiles
-// at -1 along any subset of x, y, z, wrapped -- a floor cell there has corners in this tile);
-// if any bit is set the 256 threads fill the tile's cells, two 16-byte stores each, a z row of
-// eight cells (128 bytes) per eight threads.  Cells beyond the dims (partial tiles) are skipped.
+// Fill of the cells a scatter can reach (pxl_fill_reachable_f32): one wavefront per 8x8x8 tile
+// (workgroups of one wave: a workgroup of four waves per tile took 19.8 us at 256^3 with ~5%
+// occupancy, in latency-bound rounds; one wave per tile keeps four times as many tiles in
+// flight: 10.8 us).  The
+// lanes read the tile's own occupancy words (TRI: also those of the seven tiles at -1 along any
+// subset of x, y, z, wrapped -- a floor cell there has corners in this tile); if any bit is set
+// the wave fills the tile, cell (x, y, z) = (j, lane >> 3, lane & 7) in pass j, a z row of eight
+// cells (128 bytes) per eight lanes: every cell (TRI), or only the occupied ones (nearest: a
+// sample adds only to its own occupied cell; lane i < 16 holds word i and passes it on).  Cells
+// beyond the dims (partial tiles) are skipped.
 template <bool TRI>
-__global__ __launch_bounds__(256) void k_fill_reachable(const uint32_t* __restrict__ occ, uint32_t X, uint32_t Y,
-                                                        uint32_t Z, float4* __restrict__ grad, float value) {
-  __shared__ uint32_t s_any;
+__global__ __launch_bounds__(64) void k_fill_reachable(const uint32_t* __restrict__ occ, uint32_t X, uint32_t Y,
+                                                       uint32_t Z, float4* __restrict__ grad, float value) {
   const uint32_t TX = (X + 7u) >> 3, TY = (Y + 7u) >> 3, TZ = (Z + 7u) >> 3;
   const uint32_t t = blockIdx.x;
   const uint32_t tz = t % TZ, ty = (t / TZ) % TY, tx = t / (TZ * TY);
-  if (threadIdx.x < 64) {
-    uint32_t w = 0u;
-    const int lane = threadIdx.x;
-    constexpr int kWords = TRI ? 8 * 16 : 16;  // word i: tile offset j = i >> 4 in {0, -1}^3, word i & 15
+  const int lane = threadIdx.x;
+  constexpr int kWords = TRI ? 8 * 16 : 16;  // word i: tile offset j = i >> 4 in {0, -1}^3, word i & 15
+  uint32_t w = 0u, any = 0u;
 #pragma unroll
-    for (int i0 = 0; i0 < kWords; i0 += 64) {
-      const int i = i0 + lane;
-      if (i < kWords) {
-        const int j = i >> 4;
-        const uint32_t ax = (j & 4) ? (tx == 0 ? TX - 1u : tx - 1u) : tx;
-        const uint32_t ay = (j & 2) ? (ty == 0 ? TY - 1u : ty - 1u) : ty;
-        const uint32_t az = (j & 1) ? (tz == 0 ? TZ - 1u : tz - 1u) : tz;
-        w |= occ[((ax * TY + ay) * TZ + az) * 16u + (uint32_t)(i & 15)];
-      }
+  for (int i0 = 0; i0 < kWords; i0 += 64) {
+    const int i = i0 + lane;
+    if (i < kWords) {
+      const int j = i >> 4;
+      const uint32_t ax = (j & 4) ? (tx == 0 ? TX - 1u : tx - 1u) : tx;
+      const uint32_t ay = (j & 2) ? (ty == 0 ? TY - 1u : ty - 1u) : ty;
+      const uint32_t az = (j & 1) ? (tz == 0 ? TZ - 1u : tz - 1u) : tz;
+      const uint32_t v = occ[((ax * TY + ay) * TZ + az) * 16u + (uint32_t)(i & 15)];
+      if (i0 == 0) w = v;  // lanes 0..15: this tile's own words
+      any |= v;
     }
-    const uint64_t any = __ballot(w != 0u);
-    if (lane == 0) s_any = any != 0ull ? 1u : 0u;
   }
-  __syncthreads();
-  if (s_any == 0u) return;
+  if (__ballot(any != 0u) == 0ull) return;
   const float4 v = make_float4(value, value, value, value);
+  const uint32_t y = ty * 8u + ((uint32_t)lane >> 3), z = tz * 8u + ((uint32_t)lane & 7u);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t c = threadIdx.x + 256u * (uint32_t)h;  // cell of the tile: (x, y, z) = c >> 6, (c >> 3) & 7, c & 7
-    const uint32_t x = tx * 8u + (c >> 6), y = ty * 8u + ((c >> 3) & 7u), z = tz * 8u + (c & 7u);
-    if (x < X && y < Y && z < Z) store_stream(grad + ((size_t)x * Y + y) * Z + z, v);
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t x = tx * 8u + (uint32_t)j;
+    bool reach = x < X && y < Y && z < Z;
+    if (!TRI) {  // the cell's own bit: word 2j + ((y>>2)&1) of the tile, bit (y&3)*8 + (z&7)
+      const uint32_t wj = (uint32_t)__shfl((int)w, 2 * j + ((lane >> 5) & 1));
+      reach = reach && ((wj >> ((((uint32_t)lane >> 3) & 3u) * 8u + ((uint32_t)lane & 7u))) & 1u) != 0u;
+    }
+    if (reach) store_stream(grad + ((size_t)x * Y + y) * Z + z, v);
   }
 }
 
@@ -1393,10 +1399,10 @@ int pxl_fill_reachable_f32(const pxl_grid* grid, int32_t interp, float* grad, fl
   hipStream_t s = (hipStream_t)stream;
   const uint32_t X = (uint32_t)grid->dims[0], Y = (uint32_t)grid->dims[1], Z = (uint32_t)grid->dims[2];
   if (interp == PXL_INTERP_TRILINEAR)
-    hipLaunchKernelGGL(pxl::k_fill_reachable<true>, dim3((unsigned)tiles), dim3(256), 0, s, grid->occupancy, X, Y, Z,
+    hipLaunchKernelGGL(pxl::k_fill_reachable<true>, dim3((unsigned)tiles), dim3(64), 0, s, grid->occupancy, X, Y, Z,
                        reinterpret_cast<float4*>(grad), value);
   else
-    hipLaunchKernelGGL(pxl::k_fill_reachable<false>, dim3((unsigned)tiles), dim3(256), 0, s, grid->occupancy, X, Y,
+    hipLaunchKernelGGL(pxl::k_fill_reachable<false>, dim3((unsigned)tiles), dim3(64), 0, s, grid->occupancy, X, Y,
                        Z, reinterpret_cast<float4*>(grad), value);
   return launched("pxl_fill_reachable_f32");
 }

@@ -295,9 +295,10 @@ def _cells_of_tiles(tiles, dims):
 
 @pytest.mark.parametrize("interp", ["nearest", "trilinear"])
 def test_fill_reachable_covers_every_cell_a_step_writes(interp):
-    """pxl_fill_reachable_f32 fills whole 8^3 tiles: those with an occupied cell (nearest), or
-    with one in the tile or a lower neighbour tile, wrapped (trilinear: corners reach one cell up);
-    that covers every cell the step's scatter can write, and nothing else is touched."""
+    """pxl_fill_reachable_f32 fills the occupied cells themselves (nearest), or whole 8^3 tiles
+    with an occupied cell in the tile or a lower neighbour tile, wrapped (trilinear: corners reach
+    one cell up); that covers every cell the step's scatter can write, and nothing else is
+    touched."""
     from plenoxels_amd import fill_reachable_
     say("fill reachable", interp)
     dims = (37, 34, 45)  # partial tiles on every axis
@@ -311,15 +312,16 @@ def test_fill_reachable_covers_every_cell_a_step_writes(interp):
     filled = (g.cpu() == 0.0).all(-1)
     assert bool(((g.cpu() == 0.0) | (g.cpu() == 7.0)).all())
     cells = plain.dilate_cells(occ) if interp == "trilinear" else occ  # floor cells a step keeps
-    tiles = _tiles_of(cells)
     if interp == "trilinear":
+        tiles = _tiles_of(cells)
         t = tiles.clone()
         for a in (0, 1):
             for b in (0, 1):
                 for c in (0, 1):
                     t |= torch.roll(tiles, shifts=(a, b, c), dims=(0, 1, 2))
-        tiles = t
-    assert torch.equal(filled, _cells_of_tiles(tiles, dims))
+        assert torch.equal(filled, _cells_of_tiles(t, dims))
+    else:
+        assert torch.equal(filled, occ)
     # every cell the scatter can write: the kept cell itself, or the 8 corners of a kept floor cell
     writes = cells.clone()
     if interp == "trilinear":
