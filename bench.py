@@ -284,7 +284,9 @@ def measure(S, args, peaks):
     if prof is not None:
         roof["traffic"] = prof["hbm_bytes_per_launch"]
         roof["traffic_detail"] = {k: prof[k] for k in ("read_bytes", "write_bytes", "atomic_bytes",
-                                                       "dram_read_bytes", "file") if k in prof}
+                                                       "memory_side_read_bytes", "file") if k in prof}
+        if prof.get("trace_avg_ns"):  # the profiled kernel's mean duration, beside this run's
+            roof["traffic_detail"]["profile_kernel_ms"] = round(prof["trace_avg_ns"] * 1e-6, 4)
         req = prof.get("atomic_requests")
         if req and peaks:
             rate = req / (kern_ms * 1e-3)

@@ -8,7 +8,8 @@ held-out views of res x res pixels from cameras on the radius-4 upper hemisphere
 Training (plenoxels_amd.train.Trainer, the reference loop scripts/train.py:79-184 on the fused
 kernels): grid initialised to colour 0.5 and density 0.1, trilinear lookup, Adam lr 0.1 with FP32
 state, TV weight 1e-5, no beta term, progressive growing as in the reference, `rays` random rays
-per iteration (rays // views per camera), early termination at t_stop 1e-4.  Runs once per
+per iteration (each from a uniformly drawn camera; --per-camera 1: rays // views from every camera, the
+reference's batching), early termination at t_stop 1e-4.  Runs once per
 storage dtype (BF16 and FP32 by default).
 
 Reports (one JSON line, also written to --out): ground-truth render time; the fused FP32 forward
@@ -105,6 +106,9 @@ def main(argv=None):
     ap.add_argument("--res", type=int, default=800)
     ap.add_argument("--iters", type=int, default=2000)
     ap.add_argument("--rays", type=int, default=1 << 19)
+    ap.add_argument("--per-camera", type=int, default=0,
+                    help="1: rays // views rays from every camera per iteration (the reference's batching); "
+                         "0: exactly --rays rays per iteration, each from a uniformly drawn camera")
     ap.add_argument("--samples", type=int, default=512)
     ap.add_argument("--near", type=float, default=2.0)
     ap.add_argument("--lr", type=float, default=0.1)
@@ -127,7 +131,9 @@ def main(argv=None):
     gmin = scenes.grid_min(X, X, X, pd)
     theta = 800 / 1111
     res = {"config": {"grid": X, "views": a.views, "heldout": a.heldout, "res": a.res, "iters": a.iters,
-                      "rays_per_iter": (a.rays // a.views) * a.views, "samples": S, "delta": delta, "near": a.near,
+                      "rays_per_iter": (a.rays // a.views) * a.views if a.per_camera else a.rays,
+                      "ray_batching": "rays // views per camera" if a.per_camera else "uniformly drawn camera per ray",
+                      "samples": S, "delta": delta, "near": a.near,
                       "lr": a.lr, "tv": a.tv, "beta": 0.0, "t_stop": a.t_stop, "interp": "trilinear",
                       "progressive_growing": bool(a.progressive), "init": "colour 0.5, density 0.1",
                       "corner_blocks": bool(a.corner_blocks), "skip_empty": bool(a.skip_empty)}}
@@ -184,10 +190,12 @@ def main(argv=None):
     ev_o, ev_d, ev_t = plain.camera_rays(imgs_train, c2w_train, theta, ev_cam, ev_uv[:, 0].contiguous(),
                                          ev_uv[:, 1].contiguous(), norm_mode="gpu")
     R_per_cam = a.rays // a.views
+    rays_per_iter = R_per_cam * a.views if a.per_camera else a.rays
     res["fused"] = {}
     for name in a.dtypes.split(","):
         dt = getattr(torch, name)
         tr = Trainer(imgs_dev, c2w_train, theta, gridsize=[X] * 3, points_distance=pd, number_of_rays=R_per_cam,
+                     total_rays=None if a.per_camera else a.rays,
                      num_samples=S, delta_step=delta, lr=a.lr, tv=a.tv, beta=0.0, device=dev,
                      progressive_growing=bool(a.progressive), interp="trilinear", storage_dtype=dt,
                      init_grid=init.to(dev), near=a.near, t_stop=a.t_stop, corner_blocks=bool(a.corner_blocks),
@@ -218,7 +226,7 @@ def main(argv=None):
                                       chunk=4096, **kw)
         loss64 = float(np.mean((rgba64 - ev_t.double().numpy()) ** 2))
         r = {"wall_s": round(wall, 2), "ms_per_iter": round(wall / a.iters * 1e3, 3),
-             "rays_per_s": round(a.iters * R_per_cam * a.views / wall, 1), "final_loss": hist[-1][1] if hist else None,
+             "rays_per_s": round(a.iters * rays_per_iter / wall, 1), "rays_per_iter": rays_per_iter, "final_loss": hist[-1][1] if hist else None,
              "heldout_psnr_rgb_init": round(p0, 3),
              "heldout_psnr_rgb": round(psnr(held[..., :3], held_dev[..., :3]), 3),
              "heldout_psnr_rgba": round(psnr(held, held_dev), 3),

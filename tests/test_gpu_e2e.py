@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 def test_e2e_small_bf16_and_fp32(tmp_path):
     say("e2e 64^3, 150 iterations, bf16 + fp32")
     r = e2e_train.main(["--grid", "64", "--views", "8", "--heldout", "2", "--res", "64", "--iters", "150",
-                        "--rays", "4096", "--samples", "128", "--progressive", "0", "--plain-steps", "0",
+                        "--rays", "4096", "--per-camera", "1", "--samples", "128", "--progressive", "0", "--plain-steps", "0",
                         "--log-every", "50", "--threads", "4", "--out", str(tmp_path / "e2e.json")])
     par = r["gt_forward_parity"]
     # FP32 kernel vs FP64 oracle on the same grid and rays; the t_stop cut can differ only where
