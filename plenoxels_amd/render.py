@@ -426,8 +426,9 @@ def fill_(t, value=0.0):
 
 def fill_reachable_(grad, occupancy, interp, value=0.0):
     """The zero-fill of a training step with `occupancy` (pxl_fill_reachable_f32): grad (X,Y,Z,4)
-    float32 = value in every 8x8x8 tile holding a cell the step's scatter can write, other cells
-    untouched -- for a gradient buffer zeroed once and used with one occupancy."""
+    float32 = value at every cell the step's scatter can write (nearest: the occupied cells;
+    trilinear: every cell of the 8x8x8 tiles their corners reach), other cells untouched -- for
+    a gradient buffer zeroed once and used with one occupancy."""
     if grad.dtype != torch.float32 or grad.dim() != 4 or grad.shape[-1] != 4 or not grad.is_contiguous():
         raise ValueError("grad must be a contiguous (X,Y,Z,4) float32 tensor")
     if interp not in _INTERP:
