@@ -517,6 +517,9 @@ def main():
     }
     rec.update(main_res)
     if near is not None:
+        if cpu is not None and "nearest" in cpu:  # the nearest line's own CPU baseline
+            near["cpu_baseline"] = {"value": cpu["nearest"]["value"], "unit": "rays/s", "cores": cpu["cores"],
+                                    "kind": cpu["kind"], "sample": cpu["nearest"]["sample"]}
         rec["nearest"] = near
     print(json.dumps(rec), flush=True)
 
