@@ -122,7 +122,8 @@ def main(argv=None):
     ap.add_argument("--plain-chunk", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--log-every", type=int, default=100)
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "e2e.json"))
+    ap.add_argument("--out", default=os.path.join(os.environ.get("PXL_REPORT_DIR") or os.path.join(ROOT, "reports"),
+                                                  "e2e.json"))
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     X, S = a.grid, a.samples

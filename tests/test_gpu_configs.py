@@ -25,7 +25,9 @@ from util import grad_close, say
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+# the precision reports go to $PXL_REPORT_DIR (default: reports/ at the repository root)
+OUT = os.environ.get("PXL_REPORT_DIR") or os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                        "reports")
 
 
 def _rays(w, n=None):
