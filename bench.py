@@ -23,7 +23,10 @@ scripts/train.py:182 runs it after the timed chain).  Step times: HIP events at 
 boundaries on the stream the kernels run on; the headline is the median step.
 
 The default run also measures the same workload with nearest lookup (the reference's own
-training chain, scripts/train.py:146), reported under "nearest".
+training chain, scripts/train.py:146), reported under "nearest".  `--workload blobs64` (64^3,
+8 cameras x 64x64 even-spread rays, 128 trilinear samples, no occupancy) and `--workload view800`
+(128^3, one 800x800 view, 256 nearest samples) print the same fields for those configurations
+(dense zero-fill and reduce; no CPU baseline leg).
 
 Baselines (outside the timed region of the fused path):
   plain_gpu     the plain PyTorch chain (oracle.plain, the reference's op sequence with
@@ -48,7 +51,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 from plenoxels_amd import (CameraRays, CornerBlockPair, LossMean, MarchConfig, _lib, blocks_reduce,  # noqa: E402
-                           camera_rays, fill_reachable_, occupancy_dilate, pack_occupancy, train_step)
+                           camera_rays, fill_, fill_reachable_, occupancy_dilate, pack_occupancy, train_step)
 from plenoxels_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -64,7 +67,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--interp", default="trilinear", choices=["trilinear", "nearest"])
+    ap.add_argument("--workload", default="sparse256", choices=["sparse256", "blobs64", "view800"],
+                    help="sparse256 (the headline); blobs64 (64^3, 32,768 rays x 128 samples, trilinear) and "
+                         "view800 (128^3, one 800x800 view, 256 samples, nearest) print the same fields for theirs")
+    ap.add_argument("--interp", default=None, choices=["trilinear", "nearest"],
+                    help="lookup (default: the workload's own, trilinear for sparse256)")
     ap.add_argument("--dtype", default="float32", choices=["float32", "bfloat16", "float16"])
     ap.add_argument("--rays", type=int, default=1 << 18)
     ap.add_argument("--scatter", default="blocks", choices=["blocks", "direct"],
@@ -85,18 +92,21 @@ def parse():
 
 
 class Setup:
-    """Device-resident inputs of the sparse256 workload."""
+    """Device-resident inputs of a workload (sparse256 unless --workload says otherwise)."""
 
     def __init__(self, args, dev, interp, imgs=None):
         t0 = time.time()
-        w = scenes.make_workload("sparse256", n_rays=args.rays, interp=interp)
+        name = getattr(args, "workload", "sparse256")
+        w = scenes.make_workload(name, n_rays=args.rays if name == "sparse256" else None, interp=interp)
         self.w = w
         self.dev = dev
         dtype = getattr(torch, args.dtype)
         self.grid32 = w.grid.to(dev)
         self.grid = self.grid32 if dtype == torch.float32 else self.grid32.to(dtype)
-        pts = pack_occupancy(w.occupancy.to(dev))
-        self.occ_bits = occupancy_dilate(pts, w.dims) if w.interp == "trilinear" else pts
+        self.occ_bits = None  # blobs64 and view800 have no occupancy: dense fill and reduce
+        if w.occupancy is not None:
+            pts = pack_occupancy(w.occupancy.to(dev))
+            self.occ_bits = occupancy_dilate(pts, w.dims) if w.interp == "trilinear" else pts
         B = w.c2w.shape[0]
         H, W = w.img_hw
         if imgs is None:
@@ -119,14 +129,18 @@ class Setup:
         self.loss_mean = LossMean(dev)
         torch.cuda.synchronize()
         log(f"setup {time.time() - t0:.1f}s: grid {tuple(self.grid.shape)} {self.grid.dtype}, "
-            f"occupancy {w.occupancy.float().mean().item():.4f}, rays {R}, S {w.num_samples}, {w.interp}")
+            f"occupancy {'none' if w.occupancy is None else round(w.occupancy.float().mean().item(), 4)}, "
+            f"rays {R}, S {w.num_samples}, {w.interp}")
 
     def step(self, ev=None, ev_reduce=None):
         """One step; ev = (train start, train end) events around the train-step kernel, ev_reduce
         = (start, end) around the blocks reduce, or None (each recorded event costs the
         stream a few microseconds, so the headline loop records only step boundaries)."""
         if not self.use_blocks:  # the zero-fill of the cells this occupancy lets the scatter reach
-            fill_reachable_(self.grad, self.occ_bits, self.w.interp)
+            if self.occ_bits is None:
+                fill_(self.grad, 0.0)
+            else:
+                fill_reachable_(self.grad, self.occ_bits, self.w.interp)
         o, d, tgt = camera_rays(self.camera)
         if ev is not None:
             ev[0].record()
@@ -139,7 +153,7 @@ class Setup:
             if ev_reduce is not None:
                 ev_reduce[0].record()
             blocks_reduce(self.grid, self.pair.blocks, self.grad, occupancy=self.occ_bits, accumulate=False,
-                          clear=self.pair.other, reachable_only=True)
+                          clear=self.pair.other, reachable_only=self.occ_bits is not None)
             if ev_reduce is not None:
                 ev_reduce[1].record()
             self.pair.flip()
@@ -198,12 +212,12 @@ def algorithmic_bytes(S):
     _lib.check(L.pxl_render_fwd(ctypes.byref(g), ctypes.byref(m), ctypes.byref(r), _lib.ptr(out),
                                 _lib.ptr(aux), _lib.stream_handle(S.dev)))
     nproc = aux[:, 0].contiguous().view(torch.int32).cpu().numpy().copy()
-    occ_host = S.occ_bits.cpu().numpy().view(np.uint32).copy()
+    occ_host = None if S.occ_bits is None else S.occ_bits.cpu().numpy().view(np.uint32).copy()
     gh = _lib.Grid()
     for a in range(3):
         gh.dims[a] = w.dims[a]
     gh.channels = 4
-    gh.occupancy = occ_host.ctypes.data
+    gh.occupancy = None if occ_host is None else occ_host.ctypes.data
     oh = o.cpu().numpy().copy()
     dh = d.cpu().numpy().copy()
     counts = np.zeros(S.R, np.int64)
@@ -223,9 +237,9 @@ def profile_record(S, args):
     summary of this configuration (profiles/pmc_*.json), if any: raw TCC->EA read requests by
     size (32, 64, 128 B), non-atomic write requests by size, atomic requests (each a 64-byte
     segment executed at the memory side) and DRAM-bound reads."""
-    name = f"pmc_sparse256_{S.w.interp}_{args.dtype}_{S.scatter_key}.json"
+    name = f"pmc_{S.w.name}_{S.w.interp}_{args.dtype}_{S.scatter_key}.json"
     path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path) or args.rays != (1 << 18):
+    if not os.path.exists(path) or (S.w.name == "sparse256" and args.rays != (1 << 18)):
         return None
     with open(path) as f:
         rec = json.load(f)
@@ -308,8 +322,10 @@ def plain_gpu(S, args):
     from oracle import plain
     w = S.w
     dev = S.dev
-    occ = w.occupancy.to(dev)
-    occ_used = plain.dilate_cells(occ) if w.interp == "trilinear" else occ
+    occ_used = None
+    if w.occupancy is not None:
+        occ = w.occupancy.to(dev)
+        occ_used = plain.dilate_cells(occ) if w.interp == "trilinear" else occ
     cam = w.cam_ids.to(dev).long()
     u, v = w.uv[:, 0].to(dev).contiguous(), w.uv[:, 1].to(dev).contiguous()
     gmin = torch.tensor(w.grid_min, device=dev)
@@ -488,7 +504,7 @@ def main():
     main_res = measure(S, args, peaks)
     near = None
     S2 = None
-    if args.interp == "trilinear" and not args.no_nearest:
+    if args.workload == "sparse256" and S.w.interp == "trilinear" and not args.no_nearest:
         S2 = Setup(args, dev, "nearest", imgs=S.imgs)
         near = measure(S2, args, peaks)
         near["scatter"] = S2.scatter
@@ -498,18 +514,21 @@ def main():
         if S2 is not None:
             near["plain_gpu"] = plain_gpu(S2, args)
             near["speedup_vs_plain_gpu"] = round(near["value"] / near["plain_gpu"]["value"], 2)
-        cpu = cpu_baseline(args, S)
+        cpu = cpu_baseline(args, S) if args.workload == "sparse256" else None
     value = main_res.pop("value")
+    metric = METRIC if args.workload == "sparse256" else METRIC.replace("256^3", f"{S.w.dims[0]}^3")
     rec = {
-        "metric": METRIC, "value": value, "unit": "rays/s", "n_gpus": 1,
+        "metric": metric, "value": value, "unit": "rays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_res.pop("ms_per_step"),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"float32": "f32", "bfloat16": "bf16 storage, f32 compute",
                   "float16": "f16 storage, f32 compute"}[args.dtype],
-        "data": "synthetic (seeded blob field, value-noise occupancy, random targets)",
-        "config": {"workload": "sparse256", "grid": [256, 256, 256], "rays": S.R, "samples_per_ray": 256,
-                   "interp": args.interp, "occupancy": "~5%", "t_stop": 1e-4, "cameras": 100,
-                   "image": [800, 800], "scatter": S.scatter},
+        "data": "synthetic (seeded blob field{}, random targets)".format(
+            ", value-noise occupancy" if S.w.occupancy is not None else ""),
+        "config": {"workload": S.w.name, "grid": list(S.w.dims), "rays": S.R, "samples_per_ray": S.w.num_samples,
+                   "interp": S.w.interp, "occupancy": "~5%" if S.w.occupancy is not None else "none",
+                   "t_stop": S.w.t_stop, "cameras": int(S.w.c2w.shape[0]), "image": list(S.w.img_hw),
+                   "scatter": S.scatter},
         "roofline": main_res.pop("roofline", None),
         "cpu_baseline": cpu,
         "plain_gpu": plain,

@@ -610,6 +610,61 @@ __device__ __forceinline__ void write_items(const KParams& P, const Ray& R, int 
 // floor index), so a run is keyed on item 0 (all-ceil corner) and item NI - 1 (all-floor
 // corner) together: the floor corner alone misses the integer case, the ceil corner alone
 // confuses a sample at integer z = c with its neighbour of floor c - 1.
+// Segmented inclusive sum over the wave's lanes in lane order: a lane with `head` starts a new
+// segment.  Hillis-Steele within 16-lane rows, then the row-15 and row-31 broadcasts, with the
+// operator (v, f) o (v', f') = (f' ? v' : v + v', f | f') -- each lane ends with the sum from its
+// segment's head to itself.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ void seg_step(float4& v, int& f) {
+  const int lf = __builtin_amdgcn_update_dpp(0, f, CTRL, ROW_MASK, 0xf, false);
+  const float4 lv = make_float4(dpp<CTRL, ROW_MASK>(0.f, v.x), dpp<CTRL, ROW_MASK>(0.f, v.y),
+                                dpp<CTRL, ROW_MASK>(0.f, v.z), dpp<CTRL, ROW_MASK>(0.f, v.w));
+  v.x = f ? v.x : lv.x + v.x;  // selects, not a branch: every lane stays active for the next DPP
+  v.y = f ? v.y : lv.y + v.y;
+  v.z = f ? v.z : lv.z + v.z;
+  v.w = f ? v.w : lv.w + v.w;
+  f = f | lf;
+}
+
+__device__ __forceinline__ void wave_segmented_sum(float4& v, bool head) {
+  int f = head ? 1 : 0;
+  seg_step<DPP_ROW_SHR1>(v, f);
+  seg_step<DPP_ROW_SHR2>(v, f);
+  seg_step<DPP_ROW_SHR4>(v, f);
+  seg_step<DPP_ROW_SHR8>(v, f);
+  seg_step<DPP_ROW_BCAST15, 0xa>(v, f);
+  seg_step<DPP_ROW_BCAST31, 0xc>(v, f);
+}
+
+// Nearest staging area (ns items in sample order, one cell each): when at least a quarter of
+// the items repeat the previous item's cell (grids whose cells are longer than the sample
+// spacing: dense scenes, the pooled grids of progressive growing), each run of equal cells is
+// summed by a segmented wave scan and compacted to one item, in place.  Returns the number of
+// items left to flush.
+__device__ __forceinline__ int merge_nearest_runs(int lane, int ns, uint32_t* s_cell, float4* s_val) {
+  const bool live = lane < ns;
+  const uint32_t cell = live ? s_cell[lane] : 0xffffffffu;
+  const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xfffffffeu, (int)cell, DPP_WAVE_SHR1, 0xf, 0xf,
+                                                               false);
+  const bool dup = live && lane > 0 && cell == prev;
+  const uint64_t dups = __ballot(dup);
+  if (4 * __popcll(dups) < ns) return ns;
+  float4 v = live ? s_val[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+  wave_segmented_sum(v, !dup);
+  // a run ends where the next item starts a new one (or at the last item)
+  const bool next_dup = lane + 1 < ns && ((dups >> (lane + 1)) & 1ull) != 0ull;
+  const bool tail = live && !next_dup;
+  const uint64_t tails = __ballot(tail);
+  wave_lds_sync();  // every item is read before any is rewritten
+  if (tail) {
+    const int r = lane_rank(tails);
+    s_cell[r] = cell;
+    s_val[r] = v;
+  }
+  wave_lds_sync();
+  return __popcll(tails);
+}
+
 template <int INTERP, bool RUNS = false>
 __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, int lane, int k,
                                               bool keep, uint32_t clip, const float4& gv,
@@ -625,7 +680,9 @@ __device__ __forceinline__ void scatter_chunk(const KParams& P, const Ray& R, in
   for (int h = 0; h < nk; h += NS) {  // kept samples of rank h .. h+NS-1 per stage
     if (keep && rank >= h && rank < h + NS) write_items<INTERP>(P, R, k, clip, gv, s_cell, s_val, (rank - h) * NI);
     wave_lds_sync();
-    const int ns = min(nk - h, NS), n = ns * NI;
+    const int ns = min(nk - h, NS);
+    int n = ns * NI;
+    if constexpr (INTERP == PXL_INTERP_NEAREST) n = merge_nearest_runs(lane, ns, s_cell, s_val);
     bool by_runs = false;
     if constexpr (RUNS && INTERP == PXL_INTERP_TRILINEAR) {
       // same items as the previous sample: items 0 and NI - 1 both equal
