@@ -85,6 +85,9 @@ def parse():
     ap.add_argument("--memory-rate", action="store_true",
                     help="only run the memory-request rate microbenchmark (libpxl_probe: atomics, stores and "
                          "loads of 1, 4 or 16 dwords per 64-byte segment over small and large spans) and exit")
+    ap.add_argument("--stream-rate", action="store_true",
+                    help="only measure the device's streaming rates with torch's own kernels (copy, add, fill "
+                         "of 2 GiB float32 tensors) -- the practical HBM ceiling beside the 8 TB/s nominal -- and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a known-size reference for "
                          "rocprofv3's memory counters) and exit")
@@ -476,6 +479,32 @@ def memory_rate(dev):
                                       "bytes_touched_per_s": round(req * 4 * L / t / 1e9, 1)}), flush=True)
 
 
+def stream_rate(dev):
+    """Bytes per second of torch's own streaming kernels on 2 GiB float32 tensors: copy (read +
+    write), add (two reads + write), fill (write); the mean of 10 launches after one warm-up."""
+    n = (1 << 31) // 4
+    a = torch.rand(n, device=dev)
+    b = torch.rand(n, device=dev)
+    c = torch.empty(n, device=dev)
+
+    def rate(fn, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = _events(2)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return nbytes / (e0.elapsed_time(e1) * 1e-4) / 1e12
+
+    out = {"probe": "stream_rate", "unit": "TB/s", "bytes_per_tensor": n * 4,
+           "copy": round(rate(lambda: c.copy_(a), 2 * n * 4), 3),
+           "add": round(rate(lambda: torch.add(a, b, out=c), 3 * n * 4), 3),
+           "fill": round(rate(lambda: c.fill_(1.0), n * 4), 3)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -489,6 +518,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.memory_rate:
         memory_rate(dev)
+        return
+    if args.stream_rate:
+        stream_rate(dev)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
