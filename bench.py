@@ -26,7 +26,9 @@ The default run also measures the same workload with nearest lookup (the referen
 training chain, scripts/train.py:146), reported under "nearest".  `--workload blobs64` (64^3,
 8 cameras x 64x64 even-spread rays, 128 trilinear samples, no occupancy) and `--workload view800`
 (128^3, one 800x800 view, 256 nearest samples) print the same fields for those configurations
-(dense zero-fill and reduce; no CPU baseline leg).
+(dense zero-fill and reduce; no CPU baseline leg).  `--memory-rate` (the atomic/store/load request
+rates of the probe library) and `--stream-rate` (torch's copy/add/fill rates) measure the device
+itself and exit.
 
 Baselines (outside the timed region of the fused path):
   plain_gpu     the plain PyTorch chain (oracle.plain, the reference's op sequence with
